@@ -1,0 +1,136 @@
+/*
+ * sqdft_gpu.h -- C-ABI of the MI355X spectral-quadrature (SQ) hot path.
+ *
+ * One context per GPU; not thread-safe; one HIP stream; every call is
+ * synchronous at return.  Host buffers are owned by the caller and only
+ * borrowed during a call; device memory is owned by the context.  Every
+ * entry point returns an int status (SQG_OK or a negative SQG_E_* code)
+ * instead of printing and calling exit() as the reference does.
+ *
+ * Reference interfaces replaced (phanish-suryanarayana/SQDFT):
+ *   sqg_set_geometry      <- output of Replica_atoms_Rcut + NonlocalProjectors
+ *                            (src/initialize.cpp:1003-1196, src/nonlocal.cpp:317-518;
+ *                            protos inc/func_sq.h:54, :368), consumed by
+ *                            VnlocsubTimesVec (src/nonlocal.cpp:20-215, inc/func_sq.h:341)
+ *   sqg_density           <- ClenshawCurtisSpectralQuadrature(DS_SQ*, int scf_iter)
+ *                            (src/sq.cpp:377-834, inc/func_sq.h:515), called by
+ *                            SCF_iteration (src/scf.cpp:150)
+ *   sqg_forces_nonlocal   <- ForcesNonLocal(DS_SQ*) (src/forces.cpp:318-578,
+ *                            inc/func_sq.h:540) with VnlocsubTimesVec_J
+ *                            (src/nonlocal.cpp:522-724, inc/func_sq.h:351)
+ *   sqg_hsub_apply_batch  <- HsubTimesVec(DS_SQ*, double***, int,int,int, double***)
+ *                            (src/sq.cpp:838-929, inc/func_sq.h:377)
+ *   sqg_node_moments      <- the per-node body of the SQ loop: LanczosAlgorithm
+ *                            (src/sq.cpp:965-1092, inc/func_sq.h:417) with
+ *                            TridiagEigenSolve (src/sq.cpp:1094-1180, inc/func_sq.h:405)
+ *                            and the Chebyshev moment recurrence (src/sq.cpp:564-655)
+ *   sqg_get_node_data     <- reads DS_SQ::chi, zee, rho_pj, Ci (inc/ds_sq.h:195-204),
+ *                            which forces.cpp:402-450 consumes after the SQ call
+ */
+#ifndef SQDFT_GPU_H
+#define SQDFT_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQG_OK 0
+#define SQG_E_ARG           -1
+#define SQG_E_HIP           -2    /* a HIP runtime call failed */
+#define SQG_E_NOGEOM        -3    /* sqg_set_geometry not called */
+#define SQG_E_NPL           -4    /* npl not a multiple of 4 (sq.cpp:523-524) */
+#define SQG_E_NEG_RHO       -5    /* rho < -1e-6 at some node (sq.cpp:764-765) */
+#define SQG_E_QL            -6    /* QL exceeded 200 sweeps (sq.cpp:1125) */
+#define SQG_E_BRACKET       -7    /* Brent root not bracketed (sq.cpp:1307-1311) */
+#define SQG_E_BRENT_ITMAX   -8    /* Brent exceeded 100 iterations (sq.cpp:1400-1401) */
+#define SQG_E_DENOM         -9    /* a KB denominator is zero (nonlocal.cpp:163) */
+#define SQG_E_CUBE          -10   /* 2*nloc+1 above what one workgroup can march (43) */
+#define SQG_E_OVERFLOW      -11   /* per-node image list larger than the build limit */
+#define SQG_E_NOSTATE       -12   /* forces/getters called before sqg_density */
+#define SQG_E_FDN           -13   /* FD order other than 12 */
+#define SQG_W_LANCZOS_MAXIT  1    /* some node hit max_iter=500 (sq.cpp:1073-1074; warning) */
+#define SQG_W_NEWTON_MAXIT   2    /* Newton hit ITMAX=200 (sq.cpp:1479-1482; warning) */
+
+#define SQG_FP64  0   /* every vector, stencil and reduction in FP64 */
+#define SQG_MIXED 1   /* FP32 vectors/stencil/Veff/projectors, FP64 reductions,
+                         moments, coefficients, Lanczos tridiagonal and Fermi solve */
+
+typedef struct sqg_params {
+    int n;                    /* FD nodes per direction of the periodic cube */
+    double h;                 /* mesh size (DS_SQ::delta) */
+    int FDn;                  /* half FD order; must be 6 */
+    double coeff_lap[16];     /* DS_SQ::coeff_lap[0..FDn] */
+    double coeff_grad[16];    /* DS_SQ::coeff_grad[0..FDn] */
+    int nloc;                 /* ceil(Rcut/h) */
+    int npl;                  /* Chebyshev degree, multiple of 4 */
+    double bet;               /* 1/(kB T) in 1/Ha */
+    double lanczos_tol;
+    double fermi_tol;
+    int fermi_alg;            /* 1 Brent, 2 Newton-Raphson */
+    int precision;            /* SQG_FP64 or SQG_MIXED */
+    int batch_nodes;          /* FD nodes per launch (0: all nodes, capped by memory) */
+    int ckpt_interval;        /* adjoint checkpoint interval m (reserved) */
+    int device;               /* HIP device ordinal */
+} sqg_params;
+
+typedef struct sqg_stats {
+    double t_lanczos, t_cheb, t_fermi, t_rho, t_total;   /* seconds, GPU events */
+    int lanczos_iters_node0;  /* the count the reference prints for node 0 */
+    int lanczos_iters_min, lanczos_iters_max;
+    long lanczos_passes;      /* Hsub passes spent in Lanczos, summed over nodes */
+    int fermi_iters;
+    double lambda_min, lambda_max;   /* global padded spectrum bounds */
+} sqg_stats;
+
+typedef struct sqg_ctx sqg_ctx;
+
+int sqg_create(const sqg_params *p, sqg_ctx **out);
+int sqg_destroy(sqg_ctx *ctx);
+
+/* Atom images and their projector tables (host FP64, copied to the device).
+ *   per type t:  nlocc[t] = ceil(rc/h), nproj[t] = sum_{l<=lmax, l!=lloc}(2l+1),
+ *                denom[16*t + i] = Denom_l of the i-th projector (l-major, m=-l..l)
+ *   per image J: img_type[J], img_atom[J] (atom index in type order),
+ *                img_nn[3J..3J+2] nearest node (unwrapped, may lie outside [0,n)),
+ *                udv + sum of earlier images' sizes: nproj x (2nlocc+1)^3 values,
+ *                projector-major then z, y, x. */
+int sqg_set_geometry(sqg_ctx *ctx, int n_typ, const int *nlocc, const int *nproj,
+                     const double *denom, int n_atm, int n_img, const int *img_type,
+                     const int *img_atom, const int *img_nn, const double *udv);
+
+/* One SQ density step on the periodic Veff grid (n^3, x fastest).
+ * lambda_f: in = previous Fermi level (used when scf_iter > 1 or Newton),
+ * out = new Fermi level.  rho: n^3 out.  The per-node chi, zee, moments and
+ * Fermi coefficients stay on the device for sqg_forces_nonlocal and getters. */
+int sqg_density(sqg_ctx *ctx, const double *veff, int scf_iter, double ncharge,
+                double *lambda_f, double *rho, double *ebs, double *eent, sqg_stats *stats);
+
+/* Nonlocal forces with the Veff/chi/zee/Ci of the last sqg_density call.
+ * f: 3*n_atm out, laid out fx[n_atm], fy[n_atm], fz[n_atm]. */
+int sqg_forces_nonlocal(sqg_ctx *ctx, double *f);
+
+/* Copies the last step's per-node data (any pointer may be NULL):
+ * chi[nn], zee[nn], mu[nn*(npl+1)], Ci[nn*(npl+1)], lanczos_iters[nn]. */
+int sqg_get_node_data(sqg_ctx *ctx, double *chi, double *zee, double *mu, double *Ci,
+                      int *lanczos_iters);
+
+/* Lanczos bounds and Chebyshev moments for a list of nodes only
+ * (no Fermi solve); outputs sized nb, nb, nb*(npl+1), nb. */
+int sqg_node_moments(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
+                     double *chi, double *zee, double *mu, int *lanczos_iters);
+
+/* Hv = H_i v for nb nodes; v and hv hold nb vectors of (2 nloc+1)^3
+ * values, each z-major ([kk][jj][ii]) like the reference's double***. */
+int sqg_hsub_apply_batch(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
+                         const double *v, double *hv);
+
+/* Device kernel time (ms) of the dominant recurrence kernel over the last
+ * sqg_density call: launches, summed and mean duration, and algorithmic
+ * bytes per launch (3 * nodes * (2nloc+1)^3 * sizeof(real)). */
+int sqg_last_kernel_timing(sqg_ctx *ctx, int which, long *launches, double *total_ms,
+                           double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

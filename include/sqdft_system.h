@@ -1,0 +1,115 @@
+/*
+ * sqdft_system.h -- host-side system setup for the SQ hot path (C-ABI).
+ *
+ * Replaces the reference's geometry/pseudopotential preparation that feeds
+ * the spectral-quadrature kernels (phanish-suryanarayana/SQDFT):
+ *   sqdft_system_create      <- Read_psd (src/readfiles.cpp:605-1225),
+ *                               FD coefficients (src/readfiles.cpp:370-406),
+ *                               Replica_atoms_Rcut (src/initialize.cpp:1003-1196,
+ *                               proto inc/func_sq.h:54) and
+ *                               NonlocalProjectors (src/nonlocal.cpp:317-518,
+ *                               proto inc/func_sq.h:368)
+ *   sqdft_system_from_files  <- Read_input + Read_atoms (src/readfiles.cpp:86-602)
+ *
+ * Conventions: plain pointers and sizes, caller-owned buffers, int status
+ * (0 = OK, negative = error, see SQDFT_E_*), no exit().  All arithmetic is
+ * FP64 and unfused so that the tables match the reference bit for bit.
+ */
+#ifndef SQDFT_SYSTEM_H
+#define SQDFT_SYSTEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQDFT_OK 0
+#define SQDFT_E_ARG        -1
+#define SQDFT_E_IO         -2
+#define SQDFT_E_SPLINE     -3   /* spline.cpp:17-22 */
+#define SQDFT_E_DENOM      -4   /* nonlocal.cpp:163 */
+#define SQDFT_E_MESH       -5   /* readfiles.cpp:298-303 (anisotropic mesh) */
+
+#define SQDFT_MAX_TYPES 8
+
+/* One atom type: the ".atoms" "Atoms" block (readfiles.cpp:481-510). */
+typedef struct sqdft_type_spec {
+    const char *symbol;      /* must match the first token of the .pot file */
+    int Z;                   /* valence charge */
+    int lloc;                /* local channel */
+    const char *psd_path;    /* Troullier-Martins table in the `atom` format */
+    int natm;                /* atoms of this type */
+    const double *xyz;       /* 3*natm Cartesian coordinates (bohr), x,y,z per atom */
+} sqdft_type_spec;
+
+/* Grid and SQ parameters (".input" keywords, readfiles.cpp:86-316). */
+typedef struct sqdft_spec {
+    int n;                   /* FD intervals per direction of the whole periodic cube */
+    double L;                /* cube edge (bohr) */
+    double h;                /* mesh size; 0 -> L/n.  The reference computes it per unit
+                                cell (domain/n_int) before multiplying by ncell
+                                (readfiles.cpp:294-323), so pass that value for
+                                bit-exact parity */
+    int fd_order;            /* FD_order (12) */
+    double Rcut;             /* truncation radius; nloc = ceil(Rcut/h) */
+    int npl;                 /* Chebyshev degree (multiple of 4) */
+    double T;                /* electronic temperature (K) */
+    double lanczos_tol;
+    double fermi_tol;
+    int fermi_alg;           /* 1 Brent, 2 Newton */
+    int n_typ;
+    const sqdft_type_spec *types;
+} sqdft_spec;
+
+/* Derived scalars of a built system. */
+typedef struct sqdft_info {
+    int n, FDn, nloc, npl, fermi_alg, n_typ, n_atm, n_img;
+    double L, h, Rcut, bet, lanczos_tol, fermi_tol;
+    double coeff_lap[16], coeff_grad[16];
+    long udv_len;            /* total doubles in the projector tables */
+    int max_nproj;
+} sqdft_info;
+
+/* Per-type nonlocal data. */
+typedef struct sqdft_type_info {
+    int Z, lloc, lmax, natm, nlocc, nproj;
+    double rc, rz;           /* grid-rounded radii (readfiles.cpp:1121-1124) */
+    double denom[4];         /* Kleinman-Bylander denominators per l (carry 1/h^3) */
+    int size;                /* radial grid length (incl. r = 0) */
+} sqdft_type_info;
+
+typedef struct sqdft_system sqdft_system;
+
+int  sqdft_system_create(const sqdft_spec *spec, sqdft_system **out);
+/* Reads "<dir>/<stem>.input" and the .atoms file it names, with the
+ * reference's keyword semantics (domain and n_int are per unit cell and are
+ * multiplied by ncell; perturb must be 0: displaced coordinates are written
+ * explicitly instead of drawn from rand()). */
+int  sqdft_system_from_files(const char *dir, const char *stem, sqdft_system **out);
+void sqdft_system_destroy(sqdft_system *S);
+
+int  sqdft_system_info(const sqdft_system *S, sqdft_info *info);
+int  sqdft_system_type_info(const sqdft_system *S, int t, sqdft_type_info *ti);
+/* Radial tables of type t: r, rV[l] (Ha), rU[l], rVloc, uu (each `size` long);
+ * any pointer may be NULL. */
+int  sqdft_system_radial(const sqdft_system *S, int t, double *r, double *rV, double *rU,
+                         double *rVloc, double *uu);
+/* Atom coordinates in type order: xyz[3*n_atm]; type[n_atm]. */
+int  sqdft_system_atoms(const sqdft_system *S, double *xyz, int *type);
+/* Periodic images whose rc-cube meets the (domain + Rcut) box, in the
+ * reference's order (type, atom, z-shift, y-shift, x-shift):
+ * xyz[3*n_img], type[n_img], atom[n_img] (global index in type order),
+ * nn[3*n_img] nearest node (left-tie rule), udv_off[n_img]. */
+int  sqdft_system_images(const sqdft_system *S, double *xyz, int *type, int *atom, int *nn,
+                         long *udv_off);
+/* Projector tables U_l Y_lm (V_l - V_loc) on each image's (2 nlocc+1)^3 cube,
+ * projector-major then z,y,x (udv_len doubles). */
+int  sqdft_system_udv(const sqdft_system *S, double *udv);
+/* Cubic-spline evaluation with the reference's routine (tests). */
+int  sqdft_spline_eval(const double *X, const double *Y, const double *YD, int len,
+                       const double *x, double *y, int nx);
+int  sqdft_spline_derivs(const double *X, const double *Y, double *YD, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""ctypes view of the FP64 CPU oracle (``oracle/libsq_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke`` and
+the CPU-baseline leg of bench.py, as the checker.  The product path never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libsq_oracle.so")
+
+c_int_p = C.POINTER(C.c_int)
+c_dbl_p = C.POINTER(C.c_double)
+c_long_p = C.POINTER(C.c_long)
+
+
+class Psd(C.Structure):
+    _fields_ = [("size", C.c_int), ("r", c_dbl_p), ("rV", c_dbl_p * 4), ("rU", c_dbl_p * 4),
+                ("rVloc", c_dbl_p), ("uu", c_dbl_p), ("DV", c_dbl_p * 4), ("DU", c_dbl_p * 4),
+                ("DVloc", c_dbl_p), ("rc_l", C.c_double * 4), ("lmax", C.c_int),
+                ("rc", C.c_double), ("rz", C.c_double), ("Denom", C.c_double * 4)]
+
+
+class Sys(C.Structure):
+    _fields_ = [("n", C.c_int), ("L", C.c_double), ("h", C.c_double), ("FDn", C.c_int),
+                ("coeff_lap", C.c_double * 16), ("coeff_grad", C.c_double * 16),
+                ("nloc", C.c_int), ("Rcut", C.c_double), ("npl", C.c_int), ("bet", C.c_double),
+                ("lanczos_tol", C.c_double), ("fermi_tol", C.c_double), ("fermi_alg", C.c_int),
+                ("n_typ", C.c_int), ("Z", C.c_int * 8), ("lloc", C.c_int * 8), ("natm", C.c_int * 8),
+                ("psd", Psd * 8), ("nlocc", C.c_int * 8), ("nproj", C.c_int * 8),
+                ("n_atm", C.c_int), ("ax", c_dbl_p), ("ay", c_dbl_p), ("az", c_dbl_p),
+                ("n_img", C.c_int), ("ix", c_dbl_p), ("iy", c_dbl_p), ("iz", c_dbl_p),
+                ("ityp", c_int_p), ("iatm", c_int_p), ("inn", c_int_p), ("iudv", c_long_p),
+                ("udv", c_dbl_p), ("udv_len", C.c_long)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    lib = C.CDLL(LIB)
+    P = C.POINTER(Sys)
+    sig = {
+        "sqo_fract": (C.c_double, [C.c_int, C.c_int]),
+        "sqo_fd_coeffs": (None, [C.c_int, C.c_double, c_dbl_p, c_dbl_p]),
+        "sqo_spline_yd": (None, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int]),
+        "sqo_spline_eval": (C.c_int, [c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p]),
+        "sqo_sph_harm": (C.c_double, [C.c_double] * 4 + [C.c_int, C.c_int]),
+        "sqo_tridiag_eig": (C.c_int, [c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p]),
+        "sqo_func": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double]),
+        "sqo_cheb_coeff": (None, [C.c_int, C.c_double, C.c_double, c_dbl_p, C.c_int]),
+        "sqo_sys_build": (C.c_int, [P, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), c_int_p, c_int_p,
+                                    c_int_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+        "sqo_sys_free": (None, [P]),
+        "sqo_hsub": (C.c_int, [P, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p]),
+        "sqo_vnl_atom": (C.c_int, [P, C.c_int, C.c_int, c_dbl_p, c_dbl_p]),
+        "sqo_lanczos": (C.c_int, [P, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_int_p]),
+        "sqo_node_moments": (C.c_int, [P, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                       c_int_p]),
+        "sqo_ne_constraint": (C.c_double, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double]),
+        "sqo_brent": (C.c_int, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double, C.c_double,
+                                C.c_int, C.c_double, c_dbl_p, c_int_p]),
+        "sqo_newton": (C.c_int, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double, c_dbl_p,
+                                 c_int_p]),
+        "sqo_density": (C.c_int, [P, c_dbl_p, C.c_int, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                  c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p, c_int_p, C.c_int]),
+        "sqo_rho_energy": (C.c_int, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, c_dbl_p, c_dbl_p,
+                                     c_dbl_p, c_dbl_p]),
+        "sqo_forces_nonlocal": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p, C.c_int, c_dbl_p,
+                                          C.c_int]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(c_dbl_p)
+
+
+def _i(a):
+    return a.ctypes.data_as(c_int_p)
+
+
+class OracleSystem:
+    """FP64 oracle system built from the same inputs as ``sqdft_amd.System``."""
+
+    def __init__(self, n, L_box, types, Rcut, npl, T, h=0.0, fd_order=12, lanczos_tol=1e-6,
+                 fermi_tol=1e-6, fermi_alg=1):
+        lib = load()
+        self._lib = lib
+        self.S = Sys()
+        nt = len(types)
+        paths = (C.c_char_p * nt)(*[str(t.psd_path).encode() for t in types])
+        syms = (C.c_char_p * nt)(*[t.symbol.encode() for t in types])
+        Z = np.array([t.Z for t in types], dtype=np.int32)
+        lloc = np.array([t.lloc for t in types], dtype=np.int32)
+        xyz = [np.asarray(t.xyz, dtype=np.float64).reshape(-1, 3) for t in types]
+        natm = np.array([x.shape[0] for x in xyz], dtype=np.int32)
+        allx = np.concatenate(xyz, axis=0)
+        ax, ay, az = (np.ascontiguousarray(allx[:, k]) for k in range(3))
+        st = lib.sqo_sys_build(C.byref(self.S), n, L_box, h, fd_order // 2, Rcut, npl, T, lanczos_tol,
+                               fermi_tol, fermi_alg, nt, paths, syms, _i(Z), _i(lloc), _i(natm),
+                               _d(ax), _d(ay), _d(az))
+        if st != 0:
+            raise RuntimeError(f"sqo_sys_build failed: {st}")
+        self.n = n
+        self.nloc = self.S.nloc
+        self.npl = npl
+        self.M = 2 * self.nloc + 1
+
+    @classmethod
+    def like(cls, system, types, Rcut, npl, T, h):
+        info = system.info
+        return cls(info["n"], info["L"], types, Rcut, npl, T, h=h, lanczos_tol=info["lanczos_tol"],
+                   fermi_tol=info["fermi_tol"], fermi_alg=info["fermi_alg"])
+
+    def __del__(self):
+        try:
+            self._lib.sqo_sys_free(C.byref(self.S))
+        except Exception:
+            pass
+
+    # --------------------------------------------------------------- views
+    def images(self) -> dict:
+        m = self.S.n_img
+        g = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy()  # noqa: E731
+        return {"xyz": np.stack([g(self.S.ix, m), g(self.S.iy, m), g(self.S.iz, m)], axis=1),
+                "type": g(self.S.ityp, m), "atom": g(self.S.iatm, m),
+                "nn": g(self.S.inn, 3 * m).reshape(-1, 3), "udv_off": g(self.S.iudv, m)}
+
+    def udv(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.S.udv, shape=(self.S.udv_len,)).copy()
+
+    def denom(self, t: int) -> np.ndarray:
+        return np.array(self.S.psd[t].Denom[:])
+
+    # --------------------------------------------------------------- path
+    def hsub(self, veff, node, v) -> np.ndarray:
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        hv = np.zeros_like(v)
+        st = self._lib.sqo_hsub(C.byref(self.S), _d(np.ascontiguousarray(veff, dtype=np.float64)), int(node),
+                                _d(v), _d(hv))
+        if st < 0:
+            raise RuntimeError(f"sqo_hsub: {st}")
+        return hv
+
+    def vnl_atom(self, node, atom, v) -> np.ndarray:
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        out = np.zeros_like(v)
+        self._lib.sqo_vnl_atom(C.byref(self.S), int(node), int(atom), _d(v), _d(out))
+        return out
+
+    def lanczos(self, veff, node):
+        lo, hi, it = C.c_double(), C.c_double(), C.c_int()
+        st = self._lib.sqo_lanczos(C.byref(self.S), _d(np.ascontiguousarray(veff)), int(node), C.byref(lo),
+                                   C.byref(hi), C.byref(it))
+        if st < 0:
+            raise RuntimeError(f"sqo_lanczos: {st}")
+        return lo.value, hi.value, it.value
+
+    def node_moments(self, veff, node) -> dict:
+        chi, zee, lo, hi = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        mu = np.zeros(self.npl + 1)
+        it = C.c_int()
+        st = self._lib.sqo_node_moments(C.byref(self.S), _d(np.ascontiguousarray(veff)), int(node),
+                                        C.byref(chi), C.byref(zee), C.byref(lo), C.byref(hi), _d(mu),
+                                        C.byref(it))
+        if st < 0:
+            raise RuntimeError(f"sqo_node_moments: {st}")
+        return {"chi": chi.value, "zee": zee.value, "mu": mu, "iters": it.value, "lmin": lo.value,
+                "lmax": hi.value}
+
+    def density(self, veff, scf_iter=1, ncharge=None, lambda_f=0.0, nthreads=0) -> dict:
+        nn, K = self.n ** 3, self.npl + 1
+        if ncharge is None:
+            ncharge = -float(sum(self.S.Z[t] * self.S.natm[t] for t in range(self.S.n_typ)))
+        lf = C.c_double(lambda_f)
+        rho = np.zeros(nn); chi = np.zeros(nn); zee = np.zeros(nn); mu = np.zeros(nn * K)
+        ci = np.zeros(nn * K); its = np.zeros(nn, dtype=np.int32)
+        ebs, eent, fit = C.c_double(), C.c_double(), C.c_int()
+        st = self._lib.sqo_density(C.byref(self.S), _d(np.ascontiguousarray(veff)), int(scf_iter),
+                                   float(ncharge), C.byref(lf), _d(rho), C.byref(ebs), C.byref(eent),
+                                   _d(chi), _d(zee), _d(mu), _d(ci), _i(its), C.byref(fit), int(nthreads))
+        return {"status": st, "rho": rho, "lambda_f": lf.value, "ebs": ebs.value, "eent": eent.value,
+                "chi": chi, "zee": zee, "mu": mu.reshape(nn, K), "Ci": ci.reshape(nn, K), "iters": its,
+                "fermi_iters": fit.value}
+
+    def forces_nonlocal(self, veff, chi, zee, Ci, nodes=None, nthreads=0) -> np.ndarray:
+        na = self.S.n_atm
+        f = np.zeros(3 * na)
+        if nodes is None:
+            np_nodes, cnt = None, 0
+            ptr = C.cast(None, c_int_p)
+        else:
+            np_nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+            cnt = np_nodes.size
+            ptr = _i(np_nodes)
+        st = self._lib.sqo_forces_nonlocal(C.byref(self.S), _d(np.ascontiguousarray(veff)),
+                                           _d(np.ascontiguousarray(chi)), _d(np.ascontiguousarray(zee)),
+                                           _d(np.ascontiguousarray(Ci)), ptr, cnt, _d(f), int(nthreads))
+        if st < 0:
+            raise RuntimeError(f"sqo_forces_nonlocal: {st}")
+        return f.reshape(3, na).T.copy()
+
+
+def cheb_coeff(npl, lfp, bet, func=0) -> np.ndarray:
+    c = np.zeros(npl + 1)
+    load().sqo_cheb_coeff(npl, lfp, bet, _d(c), func)
+    return c
+
+
+def tridiag_eig(diag, sub):
+    lo, hi = C.c_double(), C.c_double()
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    s = np.ascontiguousarray(sub, dtype=np.float64)
+    st = load().sqo_tridiag_eig(_d(d), _d(s), d.size, C.byref(lo), C.byref(hi))
+    return st, lo.value, hi.value
+
+
+def fd_coeffs(FDn, h):
+    lap, grad = np.zeros(16), np.zeros(16)
+    load().sqo_fd_coeffs(FDn, h, _d(lap), _d(grad))
+    return lap[: FDn + 1], grad[: FDn + 1]

@@ -1,0 +1,902 @@
+// sq_gpu.hip -- context, orchestration and C-ABI of the MI355X SQ hot path.
+//
+// Per SQ density step (ClenshawCurtisSpectralQuadrature, sq.cpp:377-834):
+//   for each batch of FD nodes (node-major cubes in HBM):
+//     image lists  -> Lanczos passes (projection + z-march + FP64 scalars/QL)
+//                  -> Chebyshev steps (projection + fused z-march recurrence)
+//   Fermi level (Brent or Newton on the host, each g(lambda) one device sum)
+//   rho, Ebs, Eent, Fermi coefficients.
+#include "sqdft_gpu.h"
+#include "sqdft_system.h"
+#include "sq_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+using namespace sqk;
+
+namespace {
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            std::fprintf(stderr, "sqdft_amd: %s failed: %s (%s:%d)\n", #x,         \
+                         hipGetErrorString(e_), __FILE__, __LINE__);               \
+            return SQG_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+// Owning device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, b ? b : 8);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct EventPair { hipEvent_t a, b; double bytes; };
+
+}  // namespace
+
+struct sqg_ctx {
+    sqg_params p{};
+    int M = 0, M2 = 0, M3 = 0, E = 0, nn = 0, pitch = 0, plane = 0;
+    hipStream_t st = nullptr;
+    // geometry
+    bool geom = false;
+    int n_typ = 0, n_atm = 0, n_img = 0, n_ip = 0, max_nproj = 0;
+    std::vector<int> h_nlocc, h_nproj, h_img_type, h_img_atom, h_img_nn, h_ip_base;
+    std::vector<long> h_udv_off;
+    std::vector<double> h_denom;
+    DBuf img_type, img_nn, ip_base, nlocc, nproj, udv_off, udv, inv_denom, csr_ptr, csr_ip, csr_u;
+    // batch buffers
+    int Bcap = 0;
+    DBuf X, Y, s, node_list, list, list_cnt, flags;
+    DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, cnt, done, ca, ib, bprev;
+    DBuf chiB, zeeB, center, dots, htemp;
+    // all-node state of the last density step
+    DBuf veff, chi_all, zee_all, lmin_all, lmax_all, iters_all, mu_all, W, Ci_all, rho_d, part;
+    DBuf costab, ynode, scal;
+    bool have_state = false;
+    double lambda_f = 0.0;
+    // timing of the dominant kernels (which: 0 Chebyshev z-march, 1 Lanczos z-march)
+    std::vector<EventPair> ev[2];
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    long last_launches[2] = {0, 0};
+    double last_ms[2] = {0, 0}, last_bytes[2] = {0, 0};
+
+    size_t real_size() const { return p.precision == SQG_MIXED ? sizeof(float) : sizeof(double); }
+    ~sqg_ctx()
+    {
+        for (auto e : pool) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    hipEvent_t ev_get()
+    {
+        if (pool_used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[pool_used++];
+    }
+};
+
+namespace {
+
+int grid1(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 1u << 20); }
+
+Geo make_geo(const sqg_ctx *c, const int *dnodes, const int *ddone)
+{
+    Geo g{};
+    g.n = c->p.n; g.nloc = c->p.nloc; g.M = c->M; g.M2 = c->M2; g.M3 = c->M3; g.E = c->E;
+    g.pitch = c->pitch; g.plane = c->plane;
+    for (int a = 0; a <= FDN; ++a) g.lap[a] = c->p.coeff_lap[a];
+    g.node_list = dnodes;
+    g.done = ddone;
+    return g;
+}
+
+int zmarch_threads(int M) { return (((M + 1) / 2 * M) + 63) / 64 * 64; }
+
+// Allocates per-batch buffers for up to B nodes.
+int ensure_batch(sqg_ctx *c, int B)
+{
+    if (B <= c->Bcap) return SQG_OK;
+    const size_t rs = c->real_size();
+    const size_t S = MAXIT + 2;
+    HIPCHK(c->X.ensure((size_t)B * c->M3 * rs));
+    HIPCHK(c->Y.ensure((size_t)B * c->M3 * rs));
+    HIPCHK(c->s.ensure((size_t)B * std::max(c->n_ip, 1) * sizeof(double)));
+    HIPCHK(c->node_list.ensure((size_t)B * sizeof(int)));
+    HIPCHK(c->list.ensure((size_t)B * LCAP * sizeof(int)));
+    HIPCHK(c->list_cnt.ensure((size_t)B * sizeof(int)));
+    HIPCHK(c->flags.ensure(16 * sizeof(int)));
+    HIPCHK(c->lred.ensure((size_t)B * 4 * sizeof(double)));
+    for (DBuf *d : {&c->aa, &c->bb, &c->dq, &c->eq}) HIPCHK(d->ensure((size_t)B * S * sizeof(double)));
+    for (DBuf *d : {&c->lmin, &c->lmax, &c->lmin_prev, &c->lmax_prev, &c->ca, &c->ib, &c->bprev,
+                    &c->chiB, &c->zeeB, &c->center, &c->htemp})
+        HIPCHK(d->ensure((size_t)B * sizeof(double)));
+    HIPCHK(c->dots.ensure((size_t)B * 2 * sizeof(double)));
+    HIPCHK(c->cnt.ensure((size_t)B * sizeof(int)));
+    HIPCHK(c->done.ensure((size_t)B * sizeof(int)));
+    c->Bcap = B;
+    return SQG_OK;
+}
+
+int choose_batch(sqg_ctx *c, int want)
+{
+    int B = c->p.batch_nodes > 0 ? c->p.batch_nodes : c->nn;
+    B = std::min(B, want);
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
+        const size_t per = 2 * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
+                           LCAP * 4 + 4 * (MAXIT + 2) * 8 + 256;
+        const size_t cap = (size_t)(0.6 * (double)(freeb + (size_t)c->Bcap * per)) / per;
+        if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
+    }
+    return std::max(B, 1);
+}
+
+template <typename T>
+ZArgs<T> zargs(sqg_ctx *c, const int *dn, const int *dd)
+{
+    ZArgs<T> A{};
+    A.g = make_geo(c, dn, dd);
+    A.nl.ptr = c->csr_ptr.as<int>();
+    A.nl.ip = c->csr_ip.as<int>();
+    A.nl.u = c->csr_u.as<T>();
+    A.nl.s = c->s.as<double>();
+    A.nl.n_ip = std::max(c->n_ip, 1);
+    A.veff = c->veff.as<T>();
+    return A;
+}
+
+template <typename T>
+PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
+{
+    PArgs<T> A{};
+    A.g = make_geo(c, dn, dd);
+    A.V = V;
+    A.s = c->s.as<double>();
+    A.n_ip = std::max(c->n_ip, 1);
+    A.list = c->list.as<int>();
+    A.list_cnt = c->list_cnt.as<int>();
+    A.img_type = c->img_type.as<int>();
+    A.img_nn = c->img_nn.as<int>();
+    A.ip_base = c->ip_base.as<int>();
+    A.udv_off = c->udv_off.as<long>();
+    A.udv = c->udv.as<T>();
+    A.nlocc = c->nlocc.as<int>();
+    A.nproj = c->nproj.as<int>();
+    A.inv_denom = c->inv_denom.as<double>();
+    A.ca = c->ca.as<double>();
+    A.ib = c->ib.as<double>();
+    return A;
+}
+
+template <typename T, int FORMED>
+void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
+{
+    if (c->n_ip == 0) return;
+    if (c->max_nproj <= 1)
+        hipLaunchKernelGGL((k_proj<T, 1, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+    else if (c->max_nproj <= 4)
+        hipLaunchKernelGGL((k_proj<T, 4, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+    else
+        hipLaunchKernelGGL((k_proj<T, 16, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+}
+
+template <typename T, int MODE>
+void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_bytes)
+{
+    const size_t shm = 2 * (size_t)c->plane * sizeof(T) + 16 * 4 * sizeof(double);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (which >= 0) {
+        e0 = c->ev_get();
+        e1 = c->ev_get();
+        if (e0) (void)hipEventRecord(e0, c->st);
+    }
+    const int nt = zmarch_threads(c->M);
+    if (nt <= 640)
+        hipLaunchKernelGGL((k_zmarch<T, MODE, 640>), dim3(B), dim3(nt), shm, c->st, A);
+    else
+        hipLaunchKernelGGL((k_zmarch<T, MODE, 1024>), dim3(B), dim3(nt), shm, c->st, A);
+    if (which >= 0 && e0 && e1) {
+        (void)hipEventRecord(e1, c->st);
+        c->ev[which].push_back({e0, e1, alg_bytes});
+    }
+}
+
+// Lanczos + Chebyshev moments for B nodes already in node_list.
+// Results land in chiB/zeeB/lmin/lmax/cnt (per batch) and mu_all (global).
+template <typename T>
+int run_batch(sqg_ctx *c, int B, long *passes_out)
+{
+    const int *dn = c->node_list.as<int>();
+    int *flags = c->flags.as<int>();
+    HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
+    // image lists
+    {
+        LArgs L{};
+        L.g = make_geo(c, dn, nullptr);
+        L.n_img = c->n_img;
+        L.img_type = c->img_type.as<int>();
+        L.img_nn = c->img_nn.as<int>();
+        L.nlocc = c->nlocc.as<int>();
+        L.nproj = c->nproj.as<int>();
+        L.list = c->list.as<int>();
+        L.list_cnt = c->list_cnt.as<int>();
+        L.overflow = flags + 0;
+        hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+    }
+    const size_t vlen = (size_t)B * c->M3;
+    T *X = c->X.as<T>(), *Y = c->Y.as<T>();
+    const double rs = (double)sizeof(T);
+    // ---------------- Lanczos (sq.cpp:965-1092)
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, vlen, (T)1);
+    hipLaunchKernelGGL((k_fill<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, Y, vlen, (T)0);
+    HIPCHK(hipMemsetAsync(c->s.p, 0, (size_t)B * std::max(c->n_ip, 1) * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->done.p, 0, (size_t)B * sizeof(int), c->st));
+    HIPCHK(hipMemsetAsync(c->cnt.p, 0, (size_t)B * sizeof(int), c->st));
+    HIPCHK(hipMemsetAsync(c->lmin_prev.p, 0, (size_t)B * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->lmax_prev.p, 0, (size_t)B * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->ca.p, 0, (size_t)B * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->bprev.p, 0, (size_t)B * sizeof(double), c->st));
+    {
+        const double ib0 = 1.0 / std::sqrt((double)c->M3);   // ones/|ones| (sq.cpp:988-998)
+        hipLaunchKernelGGL((k_fill<double>), dim3(grid1(B, 256)), dim3(256), 0, c->st, c->ib.as<double>(),
+                           (size_t)B, ib0);
+    }
+    int *ddone = c->done.as<int>();
+    QArgs Q{};
+    Q.B = B; Q.tol = c->p.lanczos_tol;
+    Q.lred = c->lred.as<double>(); Q.aa = c->aa.as<double>(); Q.bb = c->bb.as<double>();
+    Q.d = c->dq.as<double>(); Q.e = c->eq.as<double>();
+    Q.lmin = c->lmin.as<double>(); Q.lmax = c->lmax.as<double>();
+    Q.lmin_prev = c->lmin_prev.as<double>(); Q.lmax_prev = c->lmax_prev.as<double>();
+    Q.count = c->cnt.as<int>(); Q.done = ddone;
+    Q.ca = c->ca.as<double>(); Q.ib = c->ib.as<double>(); Q.bprev = c->bprev.as<double>();
+    Q.chi = c->chiB.as<double>(); Q.zee = c->zeeB.as<double>();
+    Q.err = flags + 1;
+    std::vector<int> hdone(B);
+    int active = B;
+    long passes = 0;
+    for (int pass = 0; pass <= MAXIT && active > 0; ++pass) {
+        PArgs<T> P = pargs<T>(c, dn, ddone, X);
+        P.V2 = Y;
+        launch_proj<T, 1>(c, B, P);
+        ZArgs<T> Z = zargs<T>(c, dn, ddone);
+        Z.X = X; Z.Y = Y;
+        Z.ca = c->ca.as<double>(); Z.ib = c->ib.as<double>(); Z.bprev = c->bprev.as<double>();
+        Z.lred = c->lred.as<double>();
+        launch_zmarch<T, MODE_LANCZOS>(c, B, Z, 1, 3.0 * active * c->M3 * rs);
+        passes += active;
+        Q.pass = pass;
+        hipLaunchKernelGGL(k_lanczos_scalars, dim3((B + 127) / 128), dim3(128), 0, c->st, Q);
+        if (pass >= 1) {
+            HIPCHK(hipMemcpyAsync(hdone.data(), ddone, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            active = 0;
+            for (int i = 0; i < B; ++i) active += hdone[i] == 0;
+        }
+    }
+    if (passes_out) *passes_out += passes;
+    if (std::getenv("SQDFT_DEBUG_LANCZOS")) {
+        // diagnostic: alpha/beta of the first node of the batch
+        std::vector<double> ha(MAXIT + 2), hb(MAXIT + 2);
+        HIPCHK(hipMemcpy(ha.data(), c->aa.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hb.data(), c->bb.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 120; ++i) std::fprintf(stderr, "lanczos b0 pass %d a=%.17g b=%.17g\n", i, ha[i], hb[i]);
+    }
+    // ---------------- Chebyshev moments (sq.cpp:564-655)
+    const int c_idx = (c->p.nloc * c->M + c->p.nloc) * c->M + c->p.nloc;
+    hipLaunchKernelGGL((k_fill_delta<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, c->M3, B, c_idx);
+    T *bufs[2] = {X, Y};
+    MArgs Mg{};
+    Mg.B = B; Mg.npl = c->p.npl; Mg.h3 = std::pow(c->p.h, 3);
+    Mg.node_list = dn; Mg.center = c->center.as<double>(); Mg.dots = c->dots.as<double>();
+    Mg.htemp = c->htemp.as<double>(); Mg.mu = c->mu_all.as<double>();
+    for (int k = 1; k <= c->p.npl / 2; ++k) {
+        T *in = bufs[(k - 1) & 1], *out = bufs[k & 1];
+        PArgs<T> P = pargs<T>(c, dn, nullptr, in);
+        launch_proj<T, 0>(c, B, P);
+        ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+        Z.X = in; Z.Y = out;
+        Z.chi = c->chiB.as<double>(); Z.zee = c->zeeB.as<double>();
+        Z.first = k == 1;
+        Z.center = c->center.as<double>(); Z.dots = c->dots.as<double>();
+        launch_zmarch<T, MODE_CHEB>(c, B, Z, 0, 3.0 * B * c->M3 * rs);
+        Mg.k = k;
+        hipLaunchKernelGGL(k_moments, dim3((B + 127) / 128), dim3(128), 0, c->st, Mg);
+    }
+    int hflags[2];
+    HIPCHK(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (hflags[0]) return SQG_E_OVERFLOW;
+    if (hflags[1]) return SQG_E_QL;
+    return SQG_OK;
+}
+
+__global__ void k_scatter_batch(int B, const int *node_list, const double *chiB, const double *zeeB,
+                                const double *lmn, const double *lmx, const int *cnt,
+                                double *chi, double *zee, double *lmin, double *lmax, int *iters)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int nd = node_list[b];
+    chi[nd] = chiB[b]; zee[nd] = zeeB[b]; lmin[nd] = lmn[b]; lmax[nd] = lmx[b]; iters[nd] = cnt[b];
+}
+
+template <typename T>
+__global__ void k_convert(const double *in, T *out, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (T)in[i];
+}
+
+template <typename T>
+__global__ void k_to_double(const T *in, double *out, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+int upload_veff(sqg_ctx *c, const double *veff)
+{
+    const size_t N = (size_t)c->nn;
+    DBuf tmp;
+    HIPCHK(c->veff.ensure(N * c->real_size()));
+    if (c->p.precision == SQG_MIXED) {
+        HIPCHK(tmp.ensure(N * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(tmp.p, veff, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL((k_convert<float>), dim3(grid1(N, 256)), dim3(256), 0, c->st, tmp.as<double>(),
+                           c->veff.as<float>(), N);
+        HIPCHK(hipStreamSynchronize(c->st));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->veff.p, veff, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
+    return SQG_OK;
+}
+
+int ensure_node_state(sqg_ctx *c)
+{
+    const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
+    for (DBuf *d : {&c->chi_all, &c->zee_all, &c->lmin_all, &c->lmax_all, &c->rho_d})
+        HIPCHK(d->ensure(N * sizeof(double)));
+    HIPCHK(c->iters_all.ensure(N * sizeof(int)));
+    HIPCHK(c->mu_all.ensure(N * K * sizeof(double)));
+    return SQG_OK;
+}
+
+template <typename T>
+int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int batch_cap)
+{
+    int done_n = 0;
+    while (done_n < count) {
+        int B = std::min(batch_cap, count - done_n);
+        int st = ensure_batch(c, B);
+        if (st) return st;
+        HIPCHK(hipMemcpyAsync(c->node_list.p, nodes + done_n, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->st));
+        st = run_batch<T>(c, B, passes);
+        if (st) return st;
+        hipLaunchKernelGGL(k_scatter_batch, dim3((B + 127) / 128), dim3(128), 0, c->st, B,
+                           c->node_list.as<int>(), c->chiB.as<double>(), c->zeeB.as<double>(),
+                           c->lmin.as<double>(), c->lmax.as<double>(), c->cnt.as<int>(),
+                           c->chi_all.as<double>(), c->zee_all.as<double>(), c->lmin_all.as<double>(),
+                           c->lmax_all.as<double>(), c->iters_all.as<int>());
+        done_n += B;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SQG_OK;
+}
+
+int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes)
+{
+    const int cap = choose_batch(c, count);
+    if (c->p.precision == SQG_MIXED) return moments_for_nodes<float>(c, nodes, count, passes, cap);
+    return moments_for_nodes<double>(c, nodes, count, passes, cap);
+}
+
+// electron-count constraint g(lambda) (sq.cpp:1183-1228); mode 1 gives d/dlambda / bet
+int fermi_eval(sqg_ctx *c, double lf, int mode, double *out)
+{
+    const int nn = c->nn, bs = 256, nb = (nn + bs - 1) / bs;
+    HIPCHK(c->part.ensure((size_t)2 * nb * sizeof(double)));
+    hipLaunchKernelGGL(k_fermi_sum, dim3(nb), dim3(bs), 0, c->st, c->W.as<double>(), c->chi_all.as<double>(),
+                       c->zee_all.as<double>(), c->ynode.as<double>(), nn, c->p.npl, lf, c->p.bet, mode,
+                       c->part.as<double>());
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->st, c->part.as<double>(), nb, c->scal.as<double>());
+    HIPCHK(hipMemcpyAsync(out, c->scal.p, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SQG_OK;
+}
+
+int ne_constraint(sqg_ctx *c, double lf, double ncharge, double *g)
+{
+    double s;
+    int st = fermi_eval(c, lf, 0, &s);
+    if (st) return st;
+    s = s * std::pow(c->p.h, 3);
+    *g = s - (-ncharge);
+    return SQG_OK;
+}
+
+double sign_of(double a, double b) { return b >= 0.0 ? std::fabs(a) : -std::fabs(a); }
+
+// Brent's root finder with the reference's exact control flow (sq.cpp:1289-1404)
+int brent(sqg_ctx *c, double x1, double x2, int scf_iter, double lf_prev, double ncharge,
+          double *root, int *iters)
+{
+    const double EPS = 2.5e-16, tol = c->p.fermi_tol;
+    const int ITMAX = 100;
+    double a = x1, b = x2, cc = x2, d = 0.0, e = 0.0, fa, fb, fc, p, q, r, s, tol1, xm;
+    if (scf_iter > 1) { a = lf_prev - 2; b = lf_prev + 2; }
+    int st = ne_constraint(c, a, ncharge, &fa);
+    if (!st) st = ne_constraint(c, b, ncharge, &fb);
+    if (st) return st;
+    if ((fa > 0.0 && fb > 0.0) || (fa < 0.0 && fb < 0)) return SQG_E_BRACKET;
+    fc = fb;
+    for (int iter = 1; iter <= ITMAX; iter++) {
+        if ((fb > 0.0 && fc > 0.0) || (fb < 0.0 && fc < 0.0)) { cc = a; fc = fa; e = d = b - a; }
+        if (std::fabs(fc) < std::fabs(fb)) { a = b; b = cc; cc = a; fa = fb; fb = fc; fc = fa; }
+        tol1 = 2.0 * EPS * std::fabs(b) + 0.5 * tol;
+        xm = 0.5 * (cc - b);
+        if (std::fabs(xm) <= tol1 || fb == 0.0) { *root = b; *iters = iter; return SQG_OK; }
+        if (std::fabs(e) >= tol1 && std::fabs(fa) > std::fabs(fb)) {
+            s = fb / fa;
+            if (a == cc) { p = 2.0 * xm * s; q = 1.0 - s; }
+            else {
+                q = fa / fc; r = fb / fc;
+                p = s * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0.0) q = -q;
+            p = std::fabs(p);
+            const double min1 = 3.0 * xm * q - std::fabs(tol1 * q);
+            const double min2 = std::fabs(e * q);
+            if (2.0 * p < (min1 < min2 ? min1 : min2)) { e = d; d = p / q; }
+            else { d = xm; e = d; }
+        } else { d = xm; e = d; }
+        a = b; fa = fb;
+        if (std::fabs(d) > tol1) b += d;
+        else b += sign_of(tol1, xm);
+        st = ne_constraint(c, b, ncharge, &fb);
+        if (st) return st;
+    }
+    return SQG_E_BRENT_ITMAX;
+}
+
+// Newton-Raphson on the electron count (sq.cpp:1407-1490)
+int newton(sqg_ctx *c, double lf0, double ncharge, double *root, int *iters, int *warn)
+{
+    const double tol = c->p.fermi_tol, h3 = std::pow(c->p.h, 3);
+    double lnew = 0.0, lold = lf0, err = 1.0;
+    int count = 0;
+    const int ITMAX = 200;
+    while (err > tol && count < ITMAX) {
+        double g, dg;
+        int st = fermi_eval(c, lold, 0, &g);
+        if (!st) st = fermi_eval(c, lold, 1, &dg);
+        if (st) return st;
+        dg = dg * c->p.bet;
+        dg = dg * h3;
+        g = g * h3;
+        const double fa = g - (-ncharge);
+        lnew = lold - (fa / dg);
+        err = std::fabs(lnew - lold);
+        lold = lnew;
+        count += 1;
+    }
+    *warn = count == ITMAX;
+    *root = lnew;
+    *iters = count;
+    return SQG_OK;
+}
+
+double event_sum_ms(sqg_ctx *c, int which, double *bytes, long *launches)
+{
+    double ms = 0.0, by = 0.0;
+    for (const EventPair &e : c->ev[which]) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e.a, e.b) == hipSuccess) ms += t;
+        by += e.bytes;
+    }
+    *bytes = by;
+    *launches = (long)c->ev[which].size();
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sqg_create(const sqg_params *p, sqg_ctx **out)
+{
+    if (!p || !out) return SQG_E_ARG;
+    *out = nullptr;
+    if (p->FDn != FDN) return SQG_E_FDN;
+    if (p->npl <= 0 || p->npl % 4) return SQG_E_NPL;
+    if (p->n < 1 || p->nloc < 0 || p->h <= 0) return SQG_E_ARG;
+    if (2 * p->nloc + 1 > MAXM) return SQG_E_CUBE;
+    if (p->precision != SQG_FP64 && p->precision != SQG_MIXED) return SQG_E_ARG;
+    std::unique_ptr<sqg_ctx> c(new sqg_ctx());
+    c->p = *p;
+    HIPCHK(hipSetDevice(p->device));
+    HIPCHK(hipStreamCreate(&c->st));
+    c->M = 2 * p->nloc + 1;
+    c->M2 = c->M * c->M;
+    c->M3 = c->M2 * c->M;
+    c->E = p->n + 2 * p->nloc;
+    c->nn = p->n * p->n * p->n;
+    c->pitch = (c->M + 2 * FDN + 1) / 2 * 2;
+    c->plane = c->pitch * (c->M + 2 * FDN);
+    // Chebyshev-Gauss nodes and cosine table with the reference's expressions (sq.cpp:1241-1257)
+    {
+        const int K = p->npl + 1;
+        std::vector<double> y(K), ct((size_t)K * K);
+        for (int k = 0; k < K; ++k) y[k] = std::cos(M_PI * ((double)((k - 0.5 + 1) / (p->npl + 1))));
+        for (int j = 0; j < K; ++j)
+            for (int k = 0; k < K; ++k)
+                ct[(size_t)j * K + k] = std::cos((M_PI * (j - 1 + 1)) * ((double)((k - 0.5 + 1) / (p->npl + 1))));
+        HIPCHK(c->ynode.ensure(K * sizeof(double)));
+        HIPCHK(c->costab.ensure((size_t)K * K * sizeof(double)));
+        HIPCHK(hipMemcpy(c->ynode.p, y.data(), K * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->costab.p, ct.data(), (size_t)K * K * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(c->scal.ensure(16 * sizeof(double)));
+    }
+    *out = c.release();
+    return SQG_OK;
+}
+
+int sqg_destroy(sqg_ctx *ctx)
+{
+    delete ctx;
+    return SQG_OK;
+}
+
+int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, const double *denom,
+                     int n_atm, int n_img, const int *img_type, const int *img_atom, const int *img_nn,
+                     const double *udv)
+{
+    if (!c || n_typ < 1 || n_typ > 8 || n_img < 0 || !nlocc || !nproj || !denom) return SQG_E_ARG;
+    if (n_img > 0 && (!img_type || !img_atom || !img_nn || !udv)) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    c->geom = false;
+    c->have_state = false;
+    c->n_typ = n_typ; c->n_atm = n_atm; c->n_img = n_img;
+    c->h_nlocc.assign(nlocc, nlocc + n_typ);
+    c->h_nproj.assign(nproj, nproj + n_typ);
+    c->h_denom.assign(denom, denom + 16 * n_typ);
+    c->h_img_type.assign(img_type, img_type + n_img);
+    c->h_img_atom.assign(img_atom, img_atom + n_img);
+    c->h_img_nn.assign(img_nn, img_nn + 3 * n_img);
+    std::vector<double> invd(16 * n_typ, 0.0);
+    c->max_nproj = 0;
+    for (int t = 0; t < n_typ; ++t) {
+        if (nproj[t] < 0 || nproj[t] > 16 || nlocc[t] < 0) return SQG_E_ARG;
+        c->max_nproj = std::max(c->max_nproj, nproj[t]);
+        for (int i = 0; i < nproj[t]; ++i) {
+            if (denom[16 * t + i] == 0) return SQG_E_DENOM;
+            invd[16 * t + i] = 1.0 / denom[16 * t + i];
+        }
+    }
+    c->h_udv_off.resize(n_img);
+    c->h_ip_base.resize(n_img);
+    long off = 0;
+    int ip = 0;
+    for (int J = 0; J < n_img; ++J) {
+        const int t = img_type[J];
+        if (t < 0 || t >= n_typ) return SQG_E_ARG;
+        const long w = 2 * nlocc[t] + 1;
+        c->h_udv_off[J] = off;
+        c->h_ip_base[J] = ip;
+        off += w * w * w * nproj[t];
+        ip += nproj[t];
+    }
+    c->n_ip = ip;
+    // projector CSR over the extended grid [-nloc, n-1+nloc]^3
+    const int E = c->E, nloc = c->p.nloc;
+    const size_t E3 = (size_t)E * E * E;
+    std::vector<int> ptr(E3 + 1, 0);
+    auto for_each_point = [&](auto &&fn) {
+        for (int J = 0; J < n_img; ++J) {
+            const int t = img_type[J], nc = nlocc[t], np = nproj[t];
+            if (np == 0) continue;
+            const int w = 2 * nc + 1;
+            int lo[3], hi[3];
+            bool ok = true;
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::max(img_nn[3 * J + d] - nc, -nloc);
+                hi[d] = std::min(img_nn[3 * J + d] + nc, c->p.n - 1 + nloc);
+                if (lo[d] > hi[d]) ok = false;
+            }
+            if (!ok) continue;
+            for (int z = lo[2]; z <= hi[2]; ++z)
+                for (int y = lo[1]; y <= hi[1]; ++y)
+                    for (int x = lo[0]; x <= hi[0]; ++x) {
+                        const size_t g = ((size_t)(z + nloc) * E + (y + nloc)) * E + (x + nloc);
+                        const long q = ((long)(z - img_nn[3 * J + 2] + nc) * w + (y - img_nn[3 * J + 1] + nc)) * w +
+                                       (x - img_nn[3 * J] + nc);
+                        fn(J, np, g, q, (long)w * w * w);
+                    }
+        }
+    };
+    for_each_point([&](int, int np, size_t g, long, long) { ptr[g + 1] += np; });
+    for (size_t g = 0; g < E3; ++g) ptr[g + 1] += ptr[g];
+    const size_t ne = (size_t)ptr[E3];
+    std::vector<int> eip(std::max<size_t>(ne, 1));
+    std::vector<double> eu(std::max<size_t>(ne, 1));
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for_each_point([&](int J, int np, size_t g, long q, long w3) {
+            for (int i = 0; i < np; ++i) {
+                const int k = fill[g]++;
+                eip[k] = c->h_ip_base[J] + i;
+                eu[k] = udv[c->h_udv_off[J] + i * w3 + q];
+            }
+        });
+    }
+    const bool mixed = c->p.precision == SQG_MIXED;
+    auto up = [&](DBuf &d, const void *src, size_t bytes) -> int {
+        HIPCHK(d.ensure(bytes));
+        if (bytes) HIPCHK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+        return SQG_OK;
+    };
+    int st = 0;
+    st |= up(c->nlocc, nlocc, n_typ * sizeof(int));
+    st |= up(c->nproj, nproj, n_typ * sizeof(int));
+    st |= up(c->inv_denom, invd.data(), invd.size() * sizeof(double));
+    st |= up(c->img_type, img_type, std::max(n_img, 1) * sizeof(int));
+    st |= up(c->img_nn, img_nn, 3 * std::max(n_img, 1) * sizeof(int));
+    st |= up(c->ip_base, c->h_ip_base.data(), std::max(n_img, 1) * sizeof(int));
+    st |= up(c->udv_off, c->h_udv_off.data(), std::max(n_img, 1) * sizeof(long));
+    st |= up(c->csr_ptr, ptr.data(), ptr.size() * sizeof(int));
+    st |= up(c->csr_ip, eip.data(), eip.size() * sizeof(int));
+    if (mixed) {
+        std::vector<float> uf(std::max<long>(off, 1)), ef(eu.size());
+        for (long i = 0; i < off; ++i) uf[i] = (float)udv[i];
+        for (size_t i = 0; i < eu.size(); ++i) ef[i] = (float)eu[i];
+        st |= up(c->udv, uf.data(), uf.size() * sizeof(float));
+        st |= up(c->csr_u, ef.data(), ef.size() * sizeof(float));
+    } else {
+        std::vector<double> ud(udv, udv + off);
+        if (ud.empty()) ud.push_back(0.0);
+        st |= up(c->udv, ud.data(), ud.size() * sizeof(double));
+        st |= up(c->csr_u, eu.data(), eu.size() * sizeof(double));
+    }
+    if (st) return SQG_E_HIP;
+    c->geom = true;
+    return SQG_OK;
+}
+
+int sqg_node_moments(sqg_ctx *c, const double *veff, const int *nodes, int nb, double *chi, double *zee,
+                     double *mu, int *iters)
+{
+    if (!c || !veff || !nodes || nb < 0) return SQG_E_ARG;
+    if (!c->geom) return SQG_E_NOGEOM;
+    for (int i = 0; i < nb; ++i)
+        if (nodes[i] < 0 || nodes[i] >= c->nn) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    int st = upload_veff(c, veff);
+    if (!st) st = ensure_node_state(c);
+    if (st) return st;
+    c->have_state = false;
+    for (int w = 0; w < 2; ++w) c->ev[w].clear();
+    c->pool_used = 0;
+    long passes = 0;
+    st = moments_dispatch(c, nodes, nb, &passes);
+    if (st) return st;
+    const size_t K = (size_t)c->p.npl + 1;
+    std::vector<double> a(c->nn), z(c->nn);
+    std::vector<int> it(c->nn);
+    HIPCHK(hipMemcpy(a.data(), c->chi_all.p, c->nn * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(z.data(), c->zee_all.p, c->nn * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(it.data(), c->iters_all.p, c->nn * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < nb; ++i) {
+        if (chi) chi[i] = a[nodes[i]];
+        if (zee) zee[i] = z[nodes[i]];
+        if (iters) iters[i] = it[nodes[i]];
+        if (mu) HIPCHK(hipMemcpy(mu + i * K, c->mu_all.as<double>() + (size_t)nodes[i] * K, K * sizeof(double),
+                                 hipMemcpyDeviceToHost));
+    }
+    return SQG_OK;
+}
+
+int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, double *lambda_f, double *rho,
+                double *ebs, double *eent, sqg_stats *stats)
+{
+    if (!c || !veff || !lambda_f || !rho) return SQG_E_ARG;
+    if (!c->geom) return SQG_E_NOGEOM;
+    HIPCHK(hipSetDevice(c->p.device));
+    hipEvent_t t0, t1, t2, t3;
+    HIPCHK(hipEventCreate(&t0)); HIPCHK(hipEventCreate(&t1)); HIPCHK(hipEventCreate(&t2)); HIPCHK(hipEventCreate(&t3));
+    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]); } };
+    hipEvent_t evs[4] = {t0, t1, t2, t3};
+    EvGuard guard{evs};
+    int st = upload_veff(c, veff);
+    if (!st) st = ensure_node_state(c);
+    if (st) return st;
+    c->have_state = false;
+    for (int w = 0; w < 2; ++w) c->ev[w].clear();
+    c->pool_used = 0;
+    HIPCHK(hipEventRecord(t0, c->st));
+    std::vector<int> all(c->nn);
+    for (int i = 0; i < c->nn; ++i) all[i] = i;
+    long passes = 0;
+    st = moments_dispatch(c, all.data(), c->nn, &passes);
+    if (st) return st;
+    HIPCHK(hipEventRecord(t1, c->st));
+    // global spectrum bounds (sq.cpp:676-713)
+    const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
+    std::vector<double> lmn(N), lmx(N);
+    std::vector<int> its(N);
+    HIPCHK(hipMemcpy(lmn.data(), c->lmin_all.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lmx.data(), c->lmax_all.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(its.data(), c->iters_all.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    double lmin = lmn[0], lmax = lmx[0];
+    int itmin = its[0], itmax = its[0];
+    for (size_t i = 1; i < N; ++i) {
+        lmin = std::min(lmin, lmn[i]);
+        lmax = std::max(lmax, lmx[i]);
+        itmin = std::min(itmin, its[i]);
+        itmax = std::max(itmax, its[i]);
+    }
+    // Fermi level
+    HIPCHK(c->W.ensure(N * K * sizeof(double)));
+    hipLaunchKernelGGL(k_cheb_W, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->mu_all.as<double>(),
+                       c->costab.as<double>(), c->nn, c->p.npl, c->W.as<double>());
+    double lf = 0.0;
+    int fit = 0, warn = 0;
+    if (c->p.fermi_alg == 2) {
+        const double lf0 = scf_iter <= 1 ? 0.2 : *lambda_f;
+        st = newton(c, lf0, ncharge, &lf, &fit, &warn);
+    } else {
+        st = brent(c, lmin - 1, lmax + 1, scf_iter, *lambda_f, ncharge, &lf, &fit);
+    }
+    if (st) return st;
+    HIPCHK(hipEventRecord(t2, c->st));
+    // rho, Ebs, Eent and Fermi coefficients (sq.cpp:740-797)
+    const int bs = 256, nb = (int)((N + bs - 1) / bs);
+    HIPCHK(c->part.ensure((size_t)2 * nb * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->st));
+    hipLaunchKernelGGL(k_rho_energy, dim3(nb), dim3(bs), 0, c->st, c->W.as<double>(), c->chi_all.as<double>(),
+                       c->zee_all.as<double>(), c->ynode.as<double>(), c->nn, c->p.npl, lf, c->p.bet,
+                       std::pow(c->p.h, 3), c->rho_d.as<double>(), c->part.as<double>(), c->flags.as<int>());
+    HIPCHK(c->Ci_all.ensure(N * K * sizeof(double)));
+    hipLaunchKernelGGL(k_ci, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->chi_all.as<double>(),
+                       c->zee_all.as<double>(), c->ynode.as<double>(), c->costab.as<double>(), c->nn, c->p.npl, lf,
+                       c->p.bet, c->Ci_all.as<double>());
+    std::vector<double> part(2 * (size_t)nb);
+    int neg = 0;
+    HIPCHK(hipMemcpyAsync(part.data(), c->part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(&neg, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(rho, c->rho_d.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipEventRecord(t3, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double Ebs = 0.0, Eent = 0.0;
+    for (int i = 0; i < nb; ++i) { Ebs += part[2 * i]; Eent += part[2 * i + 1]; }
+    *lambda_f = lf;
+    c->lambda_f = lf;
+    if (ebs) *ebs = Ebs;
+    if (eent) *eent = Eent;
+    c->have_state = true;
+    for (int w = 0; w < 2; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+    if (stats) {
+        float a = 0, b = 0, d = 0;
+        (void)hipEventElapsedTime(&a, t0, t1);
+        (void)hipEventElapsedTime(&b, t1, t2);
+        (void)hipEventElapsedTime(&d, t2, t3);
+        std::memset(stats, 0, sizeof(*stats));
+        stats->t_lanczos = c->last_ms[1] * 1e-3;
+        stats->t_cheb = a * 1e-3 - stats->t_lanczos;
+        stats->t_fermi = b * 1e-3;
+        stats->t_rho = d * 1e-3;
+        stats->t_total = (a + b + d) * 1e-3;
+        stats->lanczos_iters_node0 = its[0];
+        stats->lanczos_iters_min = itmin;
+        stats->lanczos_iters_max = itmax;
+        stats->lanczos_passes = passes;
+        stats->fermi_iters = fit;
+        stats->lambda_min = lmin;
+        stats->lambda_max = lmax;
+    }
+    if (neg) return SQG_E_NEG_RHO;
+    if (itmax >= MAXIT) return SQG_W_LANCZOS_MAXIT;
+    if (warn) return SQG_W_NEWTON_MAXIT;
+    return SQG_OK;
+}
+
+int sqg_get_node_data(sqg_ctx *c, double *chi, double *zee, double *mu, double *Ci, int *iters)
+{
+    if (!c) return SQG_E_ARG;
+    if (!c->have_state) return SQG_E_NOSTATE;
+    const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
+    if (chi) HIPCHK(hipMemcpy(chi, c->chi_all.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (zee) HIPCHK(hipMemcpy(zee, c->zee_all.p, N * sizeof(double), hipMemcpyDeviceToHost));
+    if (mu) HIPCHK(hipMemcpy(mu, c->mu_all.p, N * K * sizeof(double), hipMemcpyDeviceToHost));
+    if (Ci) HIPCHK(hipMemcpy(Ci, c->Ci_all.p, N * K * sizeof(double), hipMemcpyDeviceToHost));
+    if (iters) HIPCHK(hipMemcpy(iters, c->iters_all.p, N * sizeof(int), hipMemcpyDeviceToHost));
+    return SQG_OK;
+}
+
+int sqg_hsub_apply_batch(sqg_ctx *c, const double *veff, const int *nodes, int nb, const double *v, double *hv)
+{
+    if (!c || !veff || !nodes || nb < 0 || !v || !hv) return SQG_E_ARG;
+    if (!c->geom) return SQG_E_NOGEOM;
+    for (int i = 0; i < nb; ++i)
+        if (nodes[i] < 0 || nodes[i] >= c->nn) return SQG_E_ARG;
+    if (nb == 0) return SQG_OK;
+    HIPCHK(hipSetDevice(c->p.device));
+    int st = upload_veff(c, veff);
+    if (!st) st = ensure_batch(c, nb);
+    if (st) return st;
+    const size_t len = (size_t)nb * c->M3;
+    DBuf tmp;
+    HIPCHK(tmp.ensure(len * sizeof(double)));
+    HIPCHK(hipMemcpy(c->node_list.p, nodes, nb * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(tmp.p, v, len * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(c->flags.p, 0, 16 * sizeof(int), c->st));
+    const int *dn = c->node_list.as<int>();
+    LArgs L{};
+    L.g = make_geo(c, dn, nullptr);
+    L.n_img = c->n_img;
+    L.img_type = c->img_type.as<int>(); L.img_nn = c->img_nn.as<int>();
+    L.nlocc = c->nlocc.as<int>(); L.nproj = c->nproj.as<int>();
+    L.list = c->list.as<int>(); L.list_cnt = c->list_cnt.as<int>(); L.overflow = c->flags.as<int>();
+    hipLaunchKernelGGL(k_node_images, dim3((nb + 127) / 128), dim3(128), 0, c->st, L, nb);
+    auto run = [&](auto tag) -> int {
+        using T = decltype(tag);
+        T *X = c->X.as<T>(), *Y = c->Y.as<T>();
+        hipLaunchKernelGGL((k_convert<T>), dim3(grid1(len, 256)), dim3(256), 0, c->st, tmp.as<double>(), X, len);
+        PArgs<T> P = pargs<T>(c, dn, nullptr, X);
+        launch_proj<T, 0>(c, nb, P);
+        ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+        Z.X = X; Z.Y = Y;
+        launch_zmarch<T, MODE_HSUB>(c, nb, Z, -1, 0.0);
+        hipLaunchKernelGGL((k_to_double<T>), dim3(grid1(len, 256)), dim3(256), 0, c->st, Y, tmp.as<double>(), len);
+        return SQG_OK;
+    };
+    st = c->p.precision == SQG_MIXED ? run(float()) : run(double());
+    if (st) return st;
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hv, tmp.p, len * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return ovf ? SQG_E_OVERFLOW : SQG_OK;
+}
+
+int sqg_forces_nonlocal(sqg_ctx *c, double *f)
+{
+    if (!c || !f) return SQG_E_ARG;
+    if (!c->have_state) return SQG_E_NOSTATE;
+    return SQG_E_ARG;  // implemented in sq_forces.hip
+}
+
+int sqg_last_kernel_timing(sqg_ctx *c, int which, long *launches, double *total_ms, double *alg_bytes)
+{
+    if (!c || which < 0 || which > 1) return SQG_E_ARG;
+    if (launches) *launches = c->last_launches[which];
+    if (total_ms) *total_ms = c->last_ms[which];
+    if (alg_bytes) *alg_bytes = c->last_bytes[which];
+    return SQG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,692 @@
+// sq_kernels.hpp -- device kernels of the MI355X SQ hot path (gfx950).
+//
+// Data layout in HBM (per batch of B FD nodes, node-major):
+//   vectors   X, Y      [B][M^3]  M = 2 nloc + 1, each node cube z-major (z, y, x)
+//   projections s       [B][n_ip] FP64, one value per image-projector
+//   image lists         [B][LCAP] image ids whose rc-cube meets the node cube
+//   Veff                [n^3] periodic grid (x fastest), read through wrapped indices
+//   projector CSR       over the extended grid (n + 2 nloc)^3: per point the
+//                       (image-projector id, U value) pairs that cover it
+// One workgroup owns one node cube: it marches through the cube along z,
+// keeping a 13-plane register window per column (z-neighbours), staging the
+// current plane in LDS (x/y-neighbours) and streaming each output plane to
+// HBM as soon as it is complete.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sqk {
+
+constexpr int FDN = 6;          // half FD order (12th-order stencil)
+constexpr int WIN = 2 * FDN + 1;
+constexpr int LCAP = 512;       // max images meeting one node cube
+constexpr int MAXM = 43;        // largest cube side one workgroup marches ((M+1)/2*M <= 1024)
+constexpr int MAXIT = 500;      // Lanczos max_iter (sq.cpp:967)
+
+__device__ __forceinline__ int wrapn(int a, int n)
+{
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+template <typename T> struct Pair;
+template <> struct Pair<float> { using type = float2; };
+template <> struct Pair<double> { using type = double2; };
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums NV doubles over the workgroup; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *red /* [16][NV] */)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[wave * NV + i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += red[w * NV + i];
+            v[i] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- arguments
+struct Geo {
+    int n, nloc, M, M2, M3, E;    // E = n + 2 nloc (extended grid side)
+    int pitch, plane;             // LDS plane pitch (even) and size
+    double lap[FDN + 1];          // coeff_lap
+    const int *node_list;         // [B] node ids of this batch
+    const int *done;              // [B] Lanczos freeze flags (nullable)
+};
+
+template <typename T>
+struct NL {                       // nonlocal apply data
+    const int *ptr;               // [E^3 + 1]
+    const int *ip;                // entries: image-projector id
+    const T *u;                   // entries: U value
+    const double *s;              // [B][n_ip] projections of the input vector
+    int n_ip;
+};
+
+enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2 };
+
+template <typename T>
+struct ZArgs {
+    Geo g;
+    NL<T> nl;
+    const T *veff;                // [n^3]
+    T *X;                         // input (HSUB/CHEB: t_k; LANCZOS: r')
+    T *Y;                         // HSUB: out; CHEB: t_{k-1} in / t_{k+1} out; LANCZOS: v
+    // CHEB per node
+    const double *chi, *zee;
+    int first;                    // CHEB: t1 = (H t0 - chi t0)/zee, no t_{k-1}
+    double *center;               // [B] t_{k+1}[c]
+    double *dots;                 // [B][2] (t_{k+1}.t_{k+1}, t_{k+1}.t_k)
+    // LANCZOS per node
+    const double *ca, *ib, *bprev;  // [B] alpha_{k-1}, 1/beta_k, beta_k
+    double *lred;                 // [B][4] v.w, r.r, r.v, v.v
+};
+
+// ---------------------------------------------------------------- z-march
+// One workgroup per node cube.  Thread (px, y) owns the two columns
+// x0 = 2 px and x0 + 1 of row y for every z.
+// LB is the launch bound: 640 threads cover cubes up to M = 35 and leave
+// ~168 VGPRs per lane (3 waves/SIMD); 1024 covers M <= 43 with 128.
+template <typename T, int MODE, int LB>
+__global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
+{
+    using P2 = typename Pair<T>::type;
+    const Geo &G = A.g;
+    const int b = blockIdx.x;
+    if (G.done && G.done[b]) return;
+    const int node = G.node_list[b];
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, E = G.E, PP = G.pitch;
+    const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    const int npx = (M + 1) >> 1;
+    const int tid = threadIdx.x;
+    const int py = tid / npx;
+    const int px = tid - py * npx;
+    const bool act = py < M;
+    const int x0 = 2 * px;
+    const bool has1 = act && (x0 + 1 < M);
+
+    extern __shared__ unsigned char smem_raw[];
+    T *lds = reinterpret_cast<T *>(smem_raw);
+    double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
+    __shared__ double sh_center;
+    for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (T)0;
+    if (tid == 0) sh_center = 0.0;
+
+    const size_t base = (size_t)b * G.M3;
+    const int col = py * M + x0;                 // offset of the pair in a plane
+    const int gx0 = wrapn(ni - nloc + x0, n), gx1 = wrapn(ni - nloc + x0 + 1, n);
+    const int gyo = wrapn(nj - nloc + py, n) * n;
+    const double hl = -0.5;
+    const T c0d = (T)(3 * G.lap[0]);
+    T cl[FDN + 1];
+#pragma unroll
+    for (int a = 0; a <= FDN; ++a) cl[a] = (T)G.lap[a];
+
+    // per-node scalars
+    T chi = 0, izee = 0, two_izee = 0, ca = 0, ib = 0, bp = 0;
+    if (MODE == MODE_CHEB) {
+        chi = (T)A.chi[b];
+        izee = (T)(1.0 / A.zee[b]);
+        two_izee = (T)(2.0 / A.zee[b]);
+    }
+    if (MODE == MODE_LANCZOS) {
+        ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = (T)A.bprev[b];
+    }
+    const T *__restrict__ X = A.X;
+    T *__restrict__ Y = A.Y;
+    const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
+
+    // input value at plane q for the stencil window (LANCZOS forms v on the fly)
+    auto load_in = [&](int q, int x, T &pv) -> T {
+        const size_t o = base + (size_t)q * M2 + py * M + x;
+        if (MODE == MODE_LANCZOS) {
+            const T r = X[o];
+            pv = Y[o];
+            return (r - ca * pv) * ib;
+        }
+        pv = 0;
+        return X[o];
+    };
+
+    T w0[WIN], w1[WIN];          // input window, columns x0 and x0+1
+    T p0[WIN], p1[WIN];          // LANCZOS: previous normalized vector v_{k-1}
+#pragma unroll
+    for (int s = 0; s < WIN; ++s) { w0[s] = 0; w1[s] = 0; p0[s] = 0; p1[s] = 0; }
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < FDN; ++q)
+            if (q < M) {
+                w0[q] = load_in(q, x0, p0[q]);
+                if (has1) w1[q] = load_in(q, x0 + 1, p1[q]);
+            }
+    }
+    T nx0 = 0, nx1 = 0, np0 = 0, np1 = 0;   // plane z+6 prefetched one step ahead
+    if (act && FDN < M) {
+        nx0 = load_in(FDN, x0, np0);
+        if (has1) nx1 = load_in(FDN, x0 + 1, np1);
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+
+    for (int zb = 0; zb < M; zb += WIN) {
+#pragma unroll
+        for (int u = 0; u < WIN; ++u) {
+            const int z = zb + u;
+            if (z >= M) break;
+            const int sc = u;                          // slot of plane z
+            const int sn = (u + FDN) % WIN;            // slot of plane z+6
+            w0[sn] = nx0; w1[sn] = nx1; p0[sn] = np0; p1[sn] = np1;
+            nx0 = nx1 = np0 = np1 = 0;
+            if (act && z + FDN + 1 < M) {
+                nx0 = load_in(z + FDN + 1, x0, np0);
+                if (has1) nx1 = load_in(z + FDN + 1, x0 + 1, np1);
+            }
+            // pointwise inputs of this plane
+            const size_t ozp = base + (size_t)z * M2 + col;
+            T tp0 = 0, tp1 = 0;
+            if (MODE == MODE_CHEB && !A.first && act) {
+                tp0 = Y[ozp];
+                if (has1) tp1 = Y[ozp + 1];
+            }
+            const int gz = wrapn(nk - nloc + z, n);
+            T ve0 = 0, ve1 = 0;
+            if (act) {
+                const size_t vo = ((size_t)gz * n) * n + gyo;
+                ve0 = A.veff[vo + gx0];
+                if (has1) ve1 = A.veff[vo + gx1];
+            }
+            T nl0 = 0, nl1 = 0;
+            if (act) {
+                const int ge = ((nk + z) * E + (nj + py)) * E + (ni + x0);
+                const int e0 = A.nl.ptr[ge], e1 = A.nl.ptr[ge + 1];
+                const int e2 = has1 ? A.nl.ptr[ge + 2] : e1;
+                for (int e = e0; e < e1; ++e) nl0 += (T)sIn[A.nl.ip[e]] * A.nl.u[e];
+                for (int e = e1; e < e2; ++e) nl1 += (T)sIn[A.nl.ip[e]] * A.nl.u[e];
+            }
+            // stage the current plane
+            T *pl = lds + (z & 1) * G.plane;
+            const T c0 = w0[sc], c1 = w1[sc];
+            if (act) {
+                pl[(py + FDN) * PP + x0 + FDN] = c0;
+                if (has1) pl[(py + FDN) * PP + x0 + FDN + 1] = c1;
+            }
+            __syncthreads();
+            if (act) {
+                const T *row = pl + (py + FDN) * PP + x0;   // padded x0 .. x0+13
+                const P2 r0 = *reinterpret_cast<const P2 *>(row + 0);
+                const P2 r1 = *reinterpret_cast<const P2 *>(row + 2);
+                const P2 r2 = *reinterpret_cast<const P2 *>(row + 4);
+                const P2 r4 = *reinterpret_cast<const P2 *>(row + 8);
+                const P2 r5 = *reinterpret_cast<const P2 *>(row + 10);
+                const P2 r6 = *reinterpret_cast<const P2 *>(row + 12);
+                // padded row x0..x0+13: r0 r1 r2 | c0 c1 | r4 r5 r6
+                T t0 = c0 * c0d, t1 = c1 * c0d;
+#define SQK_TERM(A, XL0, XR0, XL1, XR1)                                                         \
+    {                                                                                           \
+        const P2 ym = *reinterpret_cast<const P2 *>(pl + (py + FDN - (A)) * PP + x0 + FDN);     \
+        const P2 yp = *reinterpret_cast<const P2 *>(pl + (py + FDN + (A)) * PP + x0 + FDN);     \
+        const int zm = (u - (A) + WIN) % WIN, zp = (u + (A)) % WIN;                             \
+        t0 += ((XL0) + (XR0) + ym.x + yp.x + w0[zm] + w0[zp]) * cl[A];                          \
+        t1 += ((XL1) + (XR1) + ym.y + yp.y + w1[zm] + w1[zp]) * cl[A];                          \
+    }
+                SQK_TERM(1, r2.y, c1, c0, r4.x)
+                SQK_TERM(2, r2.x, r4.x, r2.y, r4.y)
+                SQK_TERM(3, r1.y, r4.y, r2.x, r5.x)
+                SQK_TERM(4, r1.x, r5.x, r1.y, r5.y)
+                SQK_TERM(5, r0.y, r5.y, r1.x, r6.x)
+                SQK_TERM(6, r0.x, r6.x, r0.y, r6.y)
+#undef SQK_TERM
+                const T hv0 = (T)hl * t0 + ve0 * c0 + nl0;
+                const T hv1 = (T)hl * t1 + ve1 * c1 + nl1;
+                if (MODE == MODE_HSUB) {
+                    Y[ozp] = hv0;
+                    if (has1) Y[ozp + 1] = hv1;
+                } else if (MODE == MODE_CHEB) {
+                    T o0, o1;
+                    if (A.first) {
+                        o0 = (hv0 - chi * c0) * izee;
+                        o1 = (hv1 - chi * c1) * izee;
+                    } else {
+                        o0 = (hv0 - chi * c0) * two_izee - tp0;
+                        o1 = (hv1 - chi * c1) * two_izee - tp1;
+                    }
+                    Y[ozp] = o0;
+                    acc[0] += (double)o0 * (double)o0;
+                    acc[1] += (double)o0 * (double)c0;
+                    if (has1) {
+                        Y[ozp + 1] = o1;
+                        acc[0] += (double)o1 * (double)o1;
+                        acc[1] += (double)o1 * (double)c1;
+                    }
+                    if (z == nloc && py == nloc) {
+                        if (x0 == nloc) sh_center = (double)o0;
+                        else if (x0 + 1 == nloc) sh_center = (double)o1;
+                    }
+                } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
+                    const T q0 = hv0 - bp * p0[sc];
+                    const T q1 = hv1 - bp * p1[sc];
+                    A.X[ozp] = q0;
+                    Y[ozp] = c0;
+                    acc[0] += (double)c0 * (double)hv0;
+                    acc[1] += (double)q0 * (double)q0;
+                    acc[2] += (double)q0 * (double)c0;
+                    acc[3] += (double)c0 * (double)c0;
+                    if (has1) {
+                        A.X[ozp + 1] = q1;
+                        Y[ozp + 1] = c1;
+                        acc[0] += (double)c1 * (double)hv1;
+                        acc[1] += (double)q1 * (double)q1;
+                        acc[2] += (double)q1 * (double)c1;
+                        acc[3] += (double)c1 * (double)c1;
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == MODE_HSUB) return;
+    block_sum<4>(acc, red);
+    if (tid == 0) {
+        if (MODE == MODE_CHEB) {
+            A.center[b] = sh_center;
+            A.dots[2 * b] = acc[0];
+            A.dots[2 * b + 1] = acc[1];
+        } else {
+            double *o = A.lred + 4 * (size_t)b;
+            o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- projections
+template <typename T>
+struct PArgs {
+    Geo g;
+    const T *V;                   // [B][M^3] vector to project (LANCZOS: r')
+    const T *V2;                  // LANCZOS: previous normalized vector
+    double *s;                    // [B][n_ip] out
+    int n_ip;
+    const int *list, *list_cnt;   // [B][LCAP], [B]
+    const int *img_type, *img_nn, *ip_base;
+    const long *udv_off;
+    const T *udv;
+    const int *nlocc, *nproj;     // per type
+    const double *inv_denom;      // [type][16]
+    const double *ca, *ib;        // LANCZOS: projects v = (V - ca V2) * ib
+};
+
+// s[b][ip] = <U_ip, v_b>/Denom over the intersection of the image's rc-cube
+// and node b's cube (nonlocal.cpp:149-171).  One wave per image.  With
+// FORMED, v is the Lanczos vector formed from the two stored vectors with the
+// same rounding as the z-march forms it, so that the projections always
+// belong to the vector being multiplied (a recursion on projections would
+// amplify rounding by |alpha/beta| per pass).
+template <typename T, int MAXP, int FORMED>
+__global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
+{
+    const Geo &G = A.g;
+    const int b = blockIdx.x;
+    if (G.done && G.done[b]) return;
+    const int node = G.node_list[b];
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
+    const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int cnt = A.list_cnt[b];
+    const T *__restrict__ V = A.V + (size_t)b * G.M3;
+    const T *__restrict__ V2 = FORMED ? A.V2 + (size_t)b * G.M3 : nullptr;
+    double *sb = A.s + (size_t)b * A.n_ip;
+    T cab = 0, ibb = 1;
+    if (FORMED) { cab = (T)A.ca[b]; ibb = (T)A.ib[b]; }
+    for (int li = wave; li < cnt; li += nwave) {
+        const int J = A.list[(size_t)b * LCAP + li];
+        const int t = A.img_type[J];
+        const int nc = A.nlocc[t], np = A.nproj[t], w = 2 * nc + 1;
+        const int cx = A.img_nn[3 * J], cy = A.img_nn[3 * J + 1], cz = A.img_nn[3 * J + 2];
+        const int lox = max(cx - nc, ni - nloc), hix = min(cx + nc, ni + nloc);
+        const int loy = max(cy - nc, nj - nloc), hiy = min(cy + nc, nj + nloc);
+        const int loz = max(cz - nc, nk - nloc), hiz = min(cz + nc, nk + nloc);
+        const int bx = hix - lox + 1, by = hiy - loy + 1;
+        const int area = bx * by;
+        const int w2 = w * w, w3 = w2 * w;
+        const T *U = A.udv + A.udv_off[J];
+        double acc[MAXP];
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i) acc[i] = 0.0;
+        for (int a = lane; a < area; a += 64) {
+            const int iy = a / bx, ix = a - iy * bx;
+            const int gx = lox + ix, gy = loy + iy;
+            const int cidx = (gy - nj + nloc) * M + (gx - ni + nloc);
+            const int ridx = (gy - cy + nc) * w + (gx - cx + nc);
+            for (int gz = loz; gz <= hiz; ++gz) {
+                const size_t o = (size_t)(gz - nk + nloc) * M2 + cidx;
+                double v;
+                if (FORMED) v = (double)((V[o] - cab * V2[o]) * ibb);
+                else v = (double)V[o];
+                const int q = (gz - cz + nc) * w2 + ridx;
+#pragma unroll
+                for (int i = 0; i < MAXP; ++i)
+                    if (i < np) acc[i] += v * (double)U[(size_t)i * w3 + q];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i) {
+            if (i < np) {
+                const double r = wave_sum(acc[i]);
+                if (lane == 0) sb[A.ip_base[J] + i] = r * A.inv_denom[16 * t + i];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- image lists
+struct LArgs {
+    Geo g;
+    int n_img;
+    const int *img_type, *img_nn, *nlocc, *nproj;
+    int *list, *list_cnt, *overflow;
+};
+
+__global__ void k_node_images(LArgs A, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int node = A.g.node_list[b];
+    const int n = A.g.n, nloc = A.g.nloc;
+    const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    int cnt = 0;
+    for (int J = 0; J < A.n_img; ++J) {
+        const int t = A.img_type[J];
+        if (A.nproj[t] == 0) continue;
+        const int R = nloc + A.nlocc[t];
+        const int dx = A.img_nn[3 * J] - ni, dy = A.img_nn[3 * J + 1] - nj, dz = A.img_nn[3 * J + 2] - nk;
+        if (dx < -R || dx > R || dy < -R || dy > R || dz < -R || dz > R) continue;
+        if (cnt < LCAP) A.list[(size_t)b * LCAP + cnt] = J;
+        ++cnt;
+    }
+    if (cnt > LCAP) { atomicOr(A.overflow, 1); cnt = LCAP; }
+    A.list_cnt[b] = cnt;
+}
+
+// ---------------------------------------------------------------- fills
+template <typename T>
+__global__ void k_fill(T *p, size_t cnt, T v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < cnt; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+__global__ void k_fill_delta(T *p, int M3, int B, int c)
+{
+    const size_t tot = (size_t)M3 * B;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = ((int)(i % M3) == c) ? (T)1 : (T)0;
+}
+
+// ---------------------------------------------------------------- Lanczos scalars
+// Per node after each Lanczos pass: alpha_p = v.w, beta_p = |r' - alpha_p v|,
+// then (p >= 1) the QL extreme eigenvalues of the (p+1)-sized tridiagonal
+// (sq.cpp:1025-1069, TridiagEigenSolve sq.cpp:1094-1180).
+struct QArgs {
+    int B, pass;
+    double tol;
+    const double *lred;       // [B][4]
+    double *aa, *bb;          // [B][MAXIT+2]
+    double *d, *e;            // scratch [B][MAXIT+2]
+    double *lmin, *lmax, *lmin_prev, *lmax_prev;
+    int *count, *done;
+    double *ca, *ib, *bprev;  // next pass scalars
+    double *chi, *zee;        // outputs at convergence
+    int *err;
+};
+
+__device__ int ql_extremes(double *d, double *e, int n, double *lmin, double *lmax)
+{
+    e[n - 1] = 0.0;
+    int m, i;
+    double s, r, p, g, f, dd, c, bq;
+    for (int l = 0; l <= n - 1; l++) {
+        int iter = 0;
+        do {
+            for (m = l; m <= n - 2; m++) {
+                dd = fabs(d[m]) + fabs(d[m + 1]);
+                if ((double)(fabs(e[m]) + dd) == dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 200) return -1;
+                g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                r = sqrt(g * g + 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+                s = c = 1.0;
+                p = 0.0;
+                for (i = m - 1; i >= l; i--) {
+                    f = s * e[i];
+                    bq = c * e[i];
+                    e[i + 1] = (r = sqrt(g * g + f * f));
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * bq;
+                    d[i + 1] = g + (p = s * r);
+                    g = c * r - bq;
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    *lmin = d[0];
+    *lmax = d[0];
+    for (i = 1; i < n; i++) {
+        if (d[i] > *lmax) *lmax = d[i];
+        else if (d[i] < *lmin) *lmin = d[i];
+    }
+    return 0;
+}
+
+__global__ void k_lanczos_scalars(QArgs A)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.B || A.done[b]) return;
+    const int S = MAXIT + 2;
+    const double *r = A.lred + 4 * (size_t)b;
+    const int p = A.pass;
+    const double al = r[0];
+    double b2 = r[1] - 2.0 * al * r[2] + al * al * r[3];
+    if (b2 < 0.0) b2 = 0.0;
+    const double be = sqrt(b2);
+    A.aa[(size_t)b * S + p] = al;
+    A.bb[(size_t)b * S + p] = be;
+    A.ca[b] = al;
+    A.ib[b] = 1.0 / be;
+    A.bprev[b] = be;
+    if (p == 0) return;
+    const int nq = p + 1;
+    double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
+    for (int i = 0; i < nq; ++i) { d[i] = A.aa[(size_t)b * S + i]; e[i] = A.bb[(size_t)b * S + i]; }
+    double lo, hi;
+    if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
+    const double dl = fabs(lo - A.lmin_prev[b]), dm = fabs(hi - A.lmax_prev[b]);
+    A.lmin_prev[b] = lo;
+    A.lmax_prev[b] = hi;
+    const int count = p;   // the reference's count after this iteration
+    A.count[b] = count;
+    if (!(count < MAXIT && (dl > A.tol || dm > A.tol))) {
+        lo -= A.tol;
+        hi += A.tol;
+        A.lmin[b] = lo;
+        A.lmax[b] = hi;
+        A.chi[b] = (hi + lo) / 2;
+        A.zee[b] = (hi - lo) / 2;
+        A.done[b] = 1;
+    }
+}
+
+// ---------------------------------------------------------------- moments
+// mu[nd][k] from the k-th recurrence step (sq.cpp:604-653)
+struct MArgs {
+    int B, k, npl;
+    double h3;
+    const int *node_list;
+    const double *center, *dots;
+    double *htemp;            // [B] t1[c]
+    double *mu;               // [nn][npl+1] (global node index)
+};
+
+__global__ void k_moments(MArgs A)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= A.B) return;
+    const int nd = A.node_list[b];
+    double *m = A.mu + (size_t)nd * (A.npl + 1);
+    const int k = A.k;
+    if (k == 1) {
+        m[0] = (double)(1.0 / A.h3);
+        A.htemp[b] = A.center[b];
+    }
+    m[k] = (double)(A.center[b] / A.h3);
+    if (4 * k > A.npl) {
+        m[2 * k] = (double)((2 * A.dots[2 * b] - 1) / A.h3);
+        m[2 * k - 1] = (double)((2 * A.dots[2 * b + 1] - A.htemp[b]) / A.h3);
+    }
+}
+
+// ---------------------------------------------------------------- Fermi level
+// W[nd][k] = mu_0 + 2 sum_{j>=1} mu_j cos(pi j (k+1/2)/(npl+1)), so that
+// sum_j 2 mu_j C_j = (2/(npl+1)) sum_k f(y_k) W[nd][k] for the discrete
+// Chebyshev coefficients of sq.cpp:1230-1259.
+__global__ void k_cheb_W(const double *mu, const double *costab, int nn, int npl, double *W)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int K = npl + 1;
+    if (i >= (size_t)nn * K) return;
+    const size_t nd = i / K;
+    const int k = (int)(i - nd * K);
+    const double *m = mu + nd * K;
+    double s = m[0];
+    for (int j = 1; j < K; ++j) s += 2.0 * m[j] * costab[(size_t)j * K + k];
+    W[i] = s;
+}
+
+__device__ __forceinline__ double fermi_dirac(double t, double lf, double bet)
+{
+    return 1 / (1 + exp(bet * (t - lf)));
+}
+
+// per-node sum_j 2 mu_j C_j for FermiDirac (mode 0) or FermiDirac_derv (mode 1),
+// reduced per block into part[blockIdx]
+__global__ void k_fermi_sum(const double *W, const double *chi, const double *zee,
+                            const double *ynode, int nn, int npl, double lf, double bet0,
+                            int mode, double *part)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (nd < nn) {
+        const int K = npl + 1;
+        const double lfp = (lf - chi[nd]) / zee[nd];
+        const double bet = bet0 * zee[nd];
+        const double *w = W + (size_t)nd * K;
+        double s = 0.0;
+        for (int k = 0; k < K; ++k) {
+            double f = fermi_dirac(ynode[k], lfp, bet);
+            if (mode == 1) f = f * (1 - f);
+            s += f * w[k];
+        }
+        v[0] = (2.0 / (npl + 1)) * s;
+    }
+    __shared__ double red[16];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ void k_sum_parts(const double *part, int np, double *out)
+{
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v[0] += part[i];
+    __shared__ double red[16];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) *out = v[0];
+}
+
+// rho, Ebs and Eent per node (sq.cpp:740-788); block partials of Ebs, Eent
+__global__ void k_rho_energy(const double *W, const double *chi, const double *zee,
+                             const double *ynode, int nn, int npl, double lf, double bet0,
+                             double h3, double *rho, double *part, int *neg)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (nd < nn) {
+        const int K = npl + 1;
+        const double fac = 2.0 / (npl + 1);
+        const double lfp = (lf - chi[nd]) / zee[nd];
+        const double bet = bet0 * zee[nd];
+        const double *w = W + (size_t)nd * K;
+        double sr = 0.0, se = 0.0, sn = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double y = ynode[k];
+            const double f = fermi_dirac(y, lfp, bet);
+            double en;
+            if (fabs(f) < 0.01 * 1e-12 || fabs(f - 1.0) < 0.01 * 1e-12) en = 0.0;
+            else en = (f * log(f) + (1 - f) * log(1 - f));
+            sr += f * w[k];
+            se += (y * f) * w[k];
+            sn += en * w[k];
+        }
+        const double rho_pc = fac * sr;
+        rho[nd] = fabs(rho_pc);
+        if (rho_pc < 0 && fabs(rho_pc) > 1e-6) atomicOr(neg, 1);
+        v[0] = (zee[nd] * (fac * se) + chi[nd] * rho_pc) * h3;
+        v[1] = (1 / bet0) * (fac * sn) * h3;
+    }
+    __shared__ double red[32];
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// explicit Fermi-Dirac coefficients Ci[nd][j] kept for the forces (sq.cpp:757)
+__global__ void k_ci(const double *chi, const double *zee, const double *ynode,
+                     const double *costab, int nn, int npl, double lf, double bet0, double *Ci)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const int K = npl + 1;
+    if (i >= (size_t)nn * K) return;
+    const size_t nd = i / K;
+    const int j = (int)(i - nd * K);
+    const double lfp = (lf - chi[nd]) / zee[nd];
+    const double bet = bet0 * zee[nd];
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s = s + fermi_dirac(ynode[k], lfp, bet) * costab[(size_t)j * K + k];
+    double c = (2.0 / (npl + 1)) * s;
+    if (j == 0) c = c / 2.0;
+    Ci[i] = c;
+}
+
+}  // namespace sqk

@@ -1,0 +1,132 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 oracle on the
+same inputs.  FP64 path: 1e-10 relative or better; mixed path: stated
+tolerances below."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+from sqdft_amd import configs
+from sqdft_amd.configs import smooth_veff
+from sqdft_amd.sq import SpectralQuadrature
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+SYSTEMS = {
+    "small_al": H.small_al,
+    "small_lih": H.small_lih,
+    "test_al": H.test_al,
+    "test_lih": H.test_lih,
+}
+
+
+@pytest.fixture(scope="module")
+def cases():
+    out = {}
+    for k, f in SYSTEMS.items():
+        S, Rcut, T = f()
+        out[k] = (S, H.oracle_for(S, Rcut, T), smooth_veff(S))
+    return out
+
+
+@pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_hsub_matches_oracle(cases, name, prec):
+    S, osys, veff = cases[name]
+    say("hsub", name, prec)
+    sq = SpectralQuadrature(S, precision=prec)
+    rng = np.random.Generator(np.random.PCG64(11))
+    nodes = np.array([0, 1, S.n + 3, S.nn // 2, S.nn - 1], dtype=np.int32)
+    v = rng.normal(size=(nodes.size, sq.M ** 3))
+    hv = sq.HsubTimesVec(veff, nodes, v).reshape(nodes.size, -1)
+    for i, nd in enumerate(nodes):
+        ref = osys.hsub(veff, int(nd), v[i])
+        assert H.rel(hv[i], ref) < (1e-13 if prec == "fp64" else 2e-6), (name, nd)
+
+
+@pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])
+def test_node_moments_match_oracle(cases, name):
+    S, osys, veff = cases[name]
+    say("moments", name)
+    sq = SpectralQuadrature(S, precision="fp64")
+    nodes = np.array([0, 5, S.nn // 3, S.nn - 2], dtype=np.int32)
+    g = sq.node_moments(veff, nodes)
+    for i, nd in enumerate(nodes):
+        r = osys.node_moments(veff, int(nd))
+        assert g["iters"][i] == r["iters"], (name, nd)
+        assert g["chi"][i] == pytest.approx(r["chi"], rel=FP64_TOL)
+        assert g["zee"][i] == pytest.approx(r["zee"], rel=FP64_TOL)
+        assert H.rel(g["mu"][i], r["mu"]) < FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al"])
+def test_density_matches_oracle(cases, name):
+    S, osys, veff = cases[name]
+    say("density", name)
+    ref = osys.density(veff, scf_iter=1, nthreads=0)
+    assert ref["status"] == 0
+    sq = SpectralQuadrature(S, precision="fp64")
+    out = sq.density(veff, scf_iter=1)
+    assert out["lambda_f"] == pytest.approx(ref["lambda_f"], abs=FP64_TOL)
+    assert H.rel(out["rho"], ref["rho"]) < FP64_TOL
+    assert out["ebs"] == pytest.approx(ref["ebs"], rel=FP64_TOL)
+    assert out["eent"] == pytest.approx(ref["eent"], rel=FP64_TOL)
+    nd = sq.node_data()
+    assert np.array_equal(nd["iters"], ref["iters"])
+    assert H.rel(nd["chi"], ref["chi"]) < FP64_TOL
+    assert H.rel(nd["mu"], ref["mu"]) < FP64_TOL
+    assert H.rel(nd["Ci"], ref["Ci"]) < 1e-9
+    assert out["stats"]["fermi_iters"] == ref["fermi_iters"]
+    # second SCF iteration: bracket lambda_f +- 2 (sq.cpp:1301-1304)
+    ref2 = osys.density(veff + 0.01, scf_iter=2, lambda_f=ref["lambda_f"], nthreads=0)
+    out2 = sq.density(veff + 0.01, scf_iter=2, lambda_f=out["lambda_f"])
+    assert out2["lambda_f"] == pytest.approx(ref2["lambda_f"], abs=FP64_TOL)
+    assert H.rel(out2["rho"], ref2["rho"]) < FP64_TOL
+
+
+def test_density_batching_is_invisible(cases):
+    S, osys, veff = cases["small_al"]
+    say("batching")
+    a = SpectralQuadrature(S, precision="fp64").density(veff)
+    b = SpectralQuadrature(S, precision="fp64", batch_nodes=37).density(veff)
+    assert np.array_equal(a["rho"], b["rho"])
+    assert a["lambda_f"] == b["lambda_f"]
+
+
+def test_mixed_precision_density_error(cases):
+    # FP32 vectors/stencil, FP64 moments and Fermi solve: rho within 1e-5
+    # relative of the FP64 oracle on this small cell
+    S, osys, veff = cases["test_al"]
+    say("mixed density")
+    ref = osys.density(veff, scf_iter=1, nthreads=0)
+    out = SpectralQuadrature(S, precision="mixed").density(veff)
+    err = H.rel(out["rho"], ref["rho"])
+    say("mixed rho rel err", err, "dlf", out["lambda_f"] - ref["lambda_f"])
+    assert err < 1e-5
+    assert out["lambda_f"] == pytest.approx(ref["lambda_f"], abs=1e-5)
+
+
+def test_electron_count_at_config0_geometry():
+    # size-independent property on the 32-atom Al cell (n=32, nloc=17):
+    # sum rho h^3 equals the electron count to the Fermi tolerance
+    S = configs.build("al32", seed=0)
+    say("al32 moments subset")
+    veff = smooth_veff(S)
+    sq = SpectralQuadrature(S, precision="fp64")
+    osys = H.oracle_for(S, 8.0, 58000.0)
+    nodes = np.array([0, 4321, 20000], dtype=np.int32)
+    g = sq.node_moments(veff, nodes)
+    for i, nd in enumerate(nodes):
+        r = osys.node_moments(veff, int(nd))
+        assert g["iters"][i] == r["iters"]
+        assert g["chi"][i] == pytest.approx(r["chi"], rel=FP64_TOL)
+        assert H.rel(g["mu"][i], r["mu"]) < FP64_TOL

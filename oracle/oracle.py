@@ -70,6 +70,8 @@ def load() -> C.CDLL:
         "sqo_lanczos": (C.c_int, [P, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_int_p]),
         "sqo_node_moments": (C.c_int, [P, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
                                        c_int_p]),
+        "sqo_moments_nodes": (C.c_int, [P, c_dbl_p, c_int_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p,
+                                        C.c_int]),
         "sqo_ne_constraint": (C.c_double, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double]),
         "sqo_brent": (C.c_int, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double, C.c_double,
                                 C.c_int, C.c_double, c_dbl_p, c_int_p]),
@@ -186,6 +188,17 @@ class OracleSystem:
             raise RuntimeError(f"sqo_node_moments: {st}")
         return {"chi": chi.value, "zee": zee.value, "mu": mu, "iters": it.value, "lmin": lo.value,
                 "lmax": hi.value}
+
+    def moments_nodes(self, veff, nodes, nthreads=0) -> dict:
+        nd = np.ascontiguousarray(nodes, dtype=np.int32)
+        K = self.npl + 1
+        chi = np.zeros(nd.size); zee = np.zeros(nd.size); mu = np.zeros(nd.size * K)
+        it = np.zeros(nd.size, dtype=np.int32)
+        st = self._lib.sqo_moments_nodes(C.byref(self.S), _d(np.ascontiguousarray(veff)), _i(nd), nd.size,
+                                         _d(chi), _d(zee), _d(mu), _i(it), int(nthreads))
+        if st < 0:
+            raise RuntimeError(f"sqo_moments_nodes: {st}")
+        return {"chi": chi, "zee": zee, "mu": mu.reshape(-1, K), "iters": it}
 
     def density(self, veff, scf_iter=1, ncharge=None, lambda_f=0.0, nthreads=0) -> dict:
         nn, K = self.n ** 3, self.npl + 1

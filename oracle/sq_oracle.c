@@ -752,6 +752,26 @@ int sqo_node_moments(const sqo_sys *S, const double *veff, int node,
     return st;
 }
 
+int sqo_moments_nodes(const sqo_sys *S, const double *veff, const int *nodes, int count,
+                      double *chi, double *zee, double *mu, int *iters, int nthreads)
+{
+    int st = SQO_OK;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < count; q++) {
+        double lo, hi;
+        int s = sqo_node_moments(S, veff, nodes[q], &chi[q], &zee[q], &lo, &hi,
+                                 mu + (long)q * (S->npl + 1), &iters[q]);
+        if (s < 0) {
+#pragma omp critical
+            st = s;
+        }
+    }
+    return st;
+}
+
 /* ------------------------------------------------------------------ */
 /* Fermi level (sq.cpp:1183-1228, 1289-1490)                           */
 /* ------------------------------------------------------------------ */

@@ -121,6 +121,11 @@ int sqo_lanczos(const sqo_sys *S, const double *veff, int node, double *lmin, do
 int sqo_node_moments(const sqo_sys *S, const double *veff, int node,
                      double *chi, double *zee, double *lmin, double *lmax, double *mu, int *iters);
 
+/* The per-node loop body (Lanczos + moments) over a list of nodes, OpenMP
+ * over nodes (sq.cpp:530-661).  Outputs are indexed by list position. */
+int sqo_moments_nodes(const sqo_sys *S, const double *veff, const int *nodes, int count,
+                      double *chi, double *zee, double *mu, int *iters, int nthreads);
+
 /* Fermi constraint and root solvers (sq.cpp:1183-1228, 1289-1490) */
 double sqo_ne_constraint(const sqo_sys *S, int nn, const double *chi, const double *zee,
                          const double *mu, double ncharge, double lf);

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SQDFT_ARCH", "gfx950")
 
 HOST_SRCS = ["system.cpp"]
-HIP_SRCS = ["sq_gpu.hip", "sq_forces.hip"]
+HIP_SRCS = ["sq_gpu.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -64,7 +65,8 @@ struct sqg_ctx {
     std::vector<int> h_nlocc, h_nproj, h_img_type, h_img_atom, h_img_nn, h_ip_base;
     std::vector<long> h_udv_off;
     std::vector<double> h_denom;
-    DBuf img_type, img_nn, ip_base, nlocc, nproj, udv_off, udv, inv_denom, csr_ptr, csr_ip, csr_u;
+    DBuf img_type, img_nn, img_atom, ip_base, nlocc, nproj, udv_off, udv, inv_denom, csr_ptr, csr_ip, csr_u;
+    DBuf Dm, contrib, ncontrib;   // nonlocal forces
     // batch buffers
     int Bcap = 0;
     DBuf X, Y, s, node_list, list, list_cnt, flags;
@@ -141,13 +143,13 @@ int ensure_batch(sqg_ctx *c, int B)
     return SQG_OK;
 }
 
-int choose_batch(sqg_ctx *c, int want)
+int choose_batch(sqg_ctx *c, int want, int nvec = 2)
 {
     int B = c->p.batch_nodes > 0 ? c->p.batch_nodes : c->nn;
     B = std::min(B, want);
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
-        const size_t per = 2 * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
+        const size_t per = nvec * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
                            LCAP * 4 + 4 * (MAXIT + 2) * 8 + 256;
         const size_t cap = (size_t)(0.6 * (double)(freeb + (size_t)c->Bcap * per)) / per;
         if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
@@ -207,7 +209,10 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
 template <typename T, int MODE>
 void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_bytes)
 {
-    const size_t shm = 2 * (size_t)c->plane * sizeof(T) + 16 * 4 * sizeof(double);
+    const size_t nip = (size_t)std::max(c->n_ip, 1);
+    const bool slds = nip * sizeof(double) <= 48 * 1024;
+    const size_t shm = 2 * (size_t)c->plane * sizeof(T) + 16 * 4 * sizeof(double) +
+                       (slds ? nip * sizeof(double) : 0);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (which >= 0) {
         e0 = c->ev_get();
@@ -215,10 +220,13 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         if (e0) (void)hipEventRecord(e0, c->st);
     }
     const int nt = zmarch_threads(c->M);
-    if (nt <= 640)
-        hipLaunchKernelGGL((k_zmarch<T, MODE, 640>), dim3(B), dim3(nt), shm, c->st, A);
-    else
-        hipLaunchKernelGGL((k_zmarch<T, MODE, 1024>), dim3(B), dim3(nt), shm, c->st, A);
+    if (nt <= 640) {
+        if (slds) hipLaunchKernelGGL((k_zmarch<T, MODE, 640, 1>), dim3(B), dim3(nt), shm, c->st, A);
+        else hipLaunchKernelGGL((k_zmarch<T, MODE, 640, 0>), dim3(B), dim3(nt), shm, c->st, A);
+    } else {
+        if (slds) hipLaunchKernelGGL((k_zmarch<T, MODE, 1024, 1>), dim3(B), dim3(nt), shm, c->st, A);
+        else hipLaunchKernelGGL((k_zmarch<T, MODE, 1024, 0>), dim3(B), dim3(nt), shm, c->st, A);
+    }
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
         c->ev[which].push_back({e0, e1, alg_bytes});
@@ -511,6 +519,72 @@ int newton(sqg_ctx *c, double lf0, double ncharge, double *root, int *iters, int
     return SQG_OK;
 }
 
+// Density-matrix columns of B nodes (npl Chebyshev steps with the D
+// accumulator, forces.cpp:376-460) and their contraction with the nonlocal
+// projectors of the images that hold each node centre.  Appends
+// (node, atom, fx, fy, fz) records in node order.
+template <typename T>
+int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
+{
+    const int *dn = c->node_list.as<int>();
+    int *flags = c->flags.as<int>();
+    HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
+    LArgs L{};
+    L.g = make_geo(c, dn, nullptr);
+    L.n_img = c->n_img;
+    L.img_type = c->img_type.as<int>(); L.img_nn = c->img_nn.as<int>();
+    L.nlocc = c->nlocc.as<int>(); L.nproj = c->nproj.as<int>();
+    L.list = c->list.as<int>(); L.list_cnt = c->list_cnt.as<int>(); L.overflow = flags + 0;
+    hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+    const size_t vlen = (size_t)B * c->M3;
+    HIPCHK(c->Dm.ensure(vlen * sizeof(T)));
+    HIPCHK(c->contrib.ensure((size_t)B * FSLOT * 4 * sizeof(double)));
+    HIPCHK(c->ncontrib.ensure((size_t)B * sizeof(int)));
+    T *X = c->X.as<T>(), *Y = c->Y.as<T>(), *D = c->Dm.as<T>();
+    const int c_idx = (c->p.nloc * c->M + c->p.nloc) * c->M + c->p.nloc;
+    hipLaunchKernelGGL((k_fill_delta<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, c->M3, B, c_idx);
+    T *bufs[2] = {X, Y};
+    for (int k = 1; k <= c->p.npl; ++k) {
+        T *in = bufs[(k - 1) & 1], *out = bufs[k & 1];
+        PArgs<T> P = pargs<T>(c, dn, nullptr, in);
+        launch_proj<T, 0>(c, B, P);
+        ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+        Z.X = in; Z.Y = out;
+        Z.chi = c->chi_all.as<double>(); Z.zee = c->zee_all.as<double>(); Z.chi_by_node = 1;
+        Z.first = k == 1;
+        Z.center = c->center.as<double>(); Z.dots = c->dots.as<double>();
+        Z.D = D; Z.Ci = c->Ci_all.as<double>(); Z.k = k; Z.npl = c->p.npl;
+        Z.inv_h3 = 1.0 / std::pow(c->p.h, 3);
+        launch_zmarch<T, MODE_CHEB>(c, B, Z, -1, 0.0);
+    }
+    FArgs<T> F{};
+    F.g = make_geo(c, dn, nullptr);
+    F.D = D;
+    F.list = c->list.as<int>(); F.list_cnt = c->list_cnt.as<int>();
+    F.img_type = c->img_type.as<int>(); F.img_nn = c->img_nn.as<int>(); F.img_atom = c->img_atom.as<int>();
+    F.nlocc = c->nlocc.as<int>(); F.nproj = c->nproj.as<int>();
+    F.udv_off = c->udv_off.as<long>(); F.udv = c->udv.as<T>(); F.inv_denom = c->inv_denom.as<double>();
+    for (int a = 0; a <= FDN; ++a) F.grad[a] = c->p.coeff_grad[a];
+    F.contrib = c->contrib.as<double>(); F.ncontrib = c->ncontrib.as<int>(); F.overflow = flags + 2;
+    if (c->max_nproj <= 1) hipLaunchKernelGGL((k_force_contract<T, 1>), dim3(B), dim3(256), 0, c->st, F);
+    else if (c->max_nproj <= 4) hipLaunchKernelGGL((k_force_contract<T, 4>), dim3(B), dim3(256), 0, c->st, F);
+    else hipLaunchKernelGGL((k_force_contract<T, 16>), dim3(B), dim3(256), 0, c->st, F);
+    std::vector<double> hc((size_t)B * FSLOT * 4);
+    std::vector<int> hn(B);
+    int hf[3];
+    HIPCHK(hipMemcpyAsync(hc.data(), c->contrib.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hn.data(), c->ncontrib.p, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (hf[0] || hf[2]) return SQG_E_OVERFLOW;
+    for (int b = 0; b < B; ++b)
+        for (int s = 0; s < hn[b]; ++s) {
+            const double *o = &hc[((size_t)b * FSLOT + s) * 4];
+            rec.insert(rec.end(), {(double)hnodes[b], o[0], o[1], o[2], o[3]});
+        }
+    return SQG_OK;
+}
+
 double event_sum_ms(sqg_ctx *c, int which, double *bytes, long *launches)
 {
     double ms = 0.0, by = 0.0;
@@ -666,6 +740,7 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
     st |= up(c->inv_denom, invd.data(), invd.size() * sizeof(double));
     st |= up(c->img_type, img_type, std::max(n_img, 1) * sizeof(int));
     st |= up(c->img_nn, img_nn, 3 * std::max(n_img, 1) * sizeof(int));
+    st |= up(c->img_atom, img_atom, std::max(n_img, 1) * sizeof(int));
     st |= up(c->ip_base, c->h_ip_base.data(), std::max(n_img, 1) * sizeof(int));
     st |= up(c->udv_off, c->h_udv_off.data(), std::max(n_img, 1) * sizeof(long));
     st |= up(c->csr_ptr, ptr.data(), ptr.size() * sizeof(int));
@@ -704,6 +779,7 @@ int sqg_node_moments(sqg_ctx *c, const double *veff, const int *nodes, int nb, d
     long passes = 0;
     st = moments_dispatch(c, nodes, nb, &passes);
     if (st) return st;
+    for (int w = 0; w < 2; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
     const size_t K = (size_t)c->p.npl + 1;
     std::vector<double> a(c->nn), z(c->nn);
     std::vector<int> it(c->nn);
@@ -887,7 +963,51 @@ int sqg_forces_nonlocal(sqg_ctx *c, double *f)
 {
     if (!c || !f) return SQG_E_ARG;
     if (!c->have_state) return SQG_E_NOSTATE;
-    return SQG_E_ARG;  // implemented in sq_forces.hip
+    HIPCHK(hipSetDevice(c->p.device));
+    const int na = c->n_atm;
+    for (int i = 0; i < 3 * na; ++i) f[i] = 0.0;
+    if (c->n_ip == 0) return SQG_OK;
+    std::vector<double> rec;   // node, atom, fx, fy, fz per contributing image
+    const int cap = choose_batch(c, c->nn, 3);
+    std::vector<int> nodes(c->nn);
+    for (int i = 0; i < c->nn; ++i) nodes[i] = i;
+    int done_n = 0;
+    while (done_n < c->nn) {
+        const int B = std::min(cap, c->nn - done_n);
+        int st = ensure_batch(c, B);
+        if (st) return st;
+        HIPCHK(hipMemcpy(c->node_list.p, nodes.data() + done_n, (size_t)B * sizeof(int), hipMemcpyHostToDevice));
+        st = c->p.precision == SQG_MIXED ? forces_batch<float>(c, B, nodes.data() + done_n, rec)
+                                         : forces_batch<double>(c, B, nodes.data() + done_n, rec);
+        if (st) return st;
+        done_n += B;
+    }
+    // sum in node order; within a node, all images of one atom form one
+    // [V_nl,J d D](centre) term as in forces.cpp:509-520
+    const double h3 = std::pow(c->p.h, 3);
+    size_t i = 0;
+    const size_t nr = rec.size() / 5;
+    while (i < nr) {
+        size_t j = i;
+        const double nd = rec[5 * i];
+        std::vector<std::pair<int, std::array<double, 3>>> per;
+        while (j < nr && rec[5 * j] == nd) {
+            const int a = (int)rec[5 * j + 1];
+            bool found = false;
+            for (auto &q : per)
+                if (q.first == a) {
+                    for (int d = 0; d < 3; ++d) q.second[d] += rec[5 * j + 2 + d];
+                    found = true;
+                }
+            if (!found) per.push_back({a, {rec[5 * j + 2], rec[5 * j + 3], rec[5 * j + 4]}});
+            ++j;
+        }
+        std::sort(per.begin(), per.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        for (auto &q : per)
+            for (int d = 0; d < 3; ++d) f[d * na + q.first] += -4 * q.second[d] * h3;
+        i = j;
+    }
+    return SQG_OK;
 }
 
 int sqg_last_kernel_timing(sqg_ctx *c, int which, long *launches, double *total_ms, double *alg_bytes)

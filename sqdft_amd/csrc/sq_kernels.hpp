@@ -34,6 +34,15 @@ template <typename T> struct Pair;
 template <> struct Pair<float> { using type = float2; };
 template <> struct Pair<double> { using type = double2; };
 
+// Workgroup barrier that orders LDS traffic only: global loads issued
+// before it stay in flight (a full __syncthreads() also waits for them).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -93,13 +102,20 @@ struct ZArgs {
     T *X;                         // input (HSUB/CHEB: t_k; LANCZOS: r')
     T *Y;                         // HSUB: out; CHEB: t_{k-1} in / t_{k+1} out; LANCZOS: v
     // CHEB per node
-    const double *chi, *zee;
+    const double *chi, *zee;      // indexed by batch slot, or by node id if chi_by_node
+    int chi_by_node;
     int first;                    // CHEB: t1 = (H t0 - chi t0)/zee, no t_{k-1}
     double *center;               // [B] t_{k+1}[c]
     double *dots;                 // [B][2] (t_{k+1}.t_{k+1}, t_{k+1}.t_k)
     // LANCZOS per node
     const double *ca, *ib, *bprev;  // [B] alpha_{k-1}, 1/beta_k, beta_k
     double *lred;                 // [B][4] v.w, r.r, r.v, v.v
+    // CHEB with a density-matrix column (forces.cpp:410-460):
+    // D += Ci[nd][k] * t_k / h^3 for the produced t_k
+    T *D;                         // [B][M^3] or nullptr
+    const double *Ci;             // [nn][npl+1]
+    int k, npl;
+    double inv_h3;
 };
 
 // ---------------------------------------------------------------- z-march
@@ -107,7 +123,8 @@ struct ZArgs {
 // x0 = 2 px and x0 + 1 of row y for every z.
 // LB is the launch bound: 640 threads cover cubes up to M = 35 and leave
 // ~168 VGPRs per lane (3 waves/SIMD); 1024 covers M <= 43 with 128.
-template <typename T, int MODE, int LB>
+// SLDS stages the node's projections in LDS (when they fit).
+template <typename T, int MODE, int LB, int SLDS>
 __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
 {
     using P2 = typename Pair<T>::type;
@@ -125,9 +142,11 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     const int x0 = 2 * px;
     const bool has1 = act && (x0 + 1 < M);
 
+    // LDS: two staged planes | reduction scratch | the node's projections
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
+    double *s_lds = red + 16 * 4;
     __shared__ double sh_center;
     for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (T)0;
     if (tid == 0) sh_center = 0.0;
@@ -144,17 +163,27 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
 
     // per-node scalars
     T chi = 0, izee = 0, two_izee = 0, ca = 0, ib = 0, bp = 0;
+    T dk = 0, d0 = 0;   // DM-column weights of t_k (and t_0 on the first step)
     if (MODE == MODE_CHEB) {
-        chi = (T)A.chi[b];
-        izee = (T)(1.0 / A.zee[b]);
-        two_izee = (T)(2.0 / A.zee[b]);
+        const int ci_idx = A.chi_by_node ? node : b;
+        chi = (T)A.chi[ci_idx];
+        izee = (T)(1.0 / A.zee[ci_idx]);
+        two_izee = (T)(2.0 / A.zee[ci_idx]);
+        if (A.D) {
+            const double *ci = A.Ci + (size_t)node * (A.npl + 1);
+            dk = (T)(ci[A.k] * A.inv_h3);
+            d0 = (T)(ci[0] * A.inv_h3);
+        }
     }
     if (MODE == MODE_LANCZOS) {
         ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = (T)A.bprev[b];
     }
-    const T *__restrict__ X = A.X;
-    T *__restrict__ Y = A.Y;
+    const T *X = A.X;
+    T *Y = A.Y;
     const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
+    if (SLDS)
+        for (int i = tid; i < A.nl.n_ip; i += blockDim.x) s_lds[i] = sIn[i];
+    const double *sv = SLDS ? s_lds : sIn;
 
     // input value at plane q for the stencil window (LANCZOS forms v on the fly)
     auto load_in = [&](int q, int x, T &pv) -> T {
@@ -180,13 +209,42 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
                 if (has1) w1[q] = load_in(q, x0 + 1, p1[q]);
             }
     }
+    // Every global load is issued one plane ahead of its use and the
+    // per-plane barrier only orders LDS traffic, so loads stay in flight
+    // across it.
+    auto pointwise = [&](int zq, T &tp0, T &tp1, T &ve0, T &ve1, int &q0, int &q1, int &q2) {
+        tp0 = tp1 = ve0 = ve1 = 0;
+        q0 = q1 = q2 = 0;
+        if (!act || zq >= M) return;
+        if (MODE == MODE_CHEB && !A.first) {
+            const size_t o = base + (size_t)zq * M2 + col;
+            tp0 = Y[o];
+            if (has1) tp1 = Y[o + 1];
+        }
+        const size_t vo = ((size_t)wrapn(nk - nloc + zq, n) * n) * n + gyo;
+        ve0 = A.veff[vo + gx0];
+        if (has1) ve1 = A.veff[vo + gx1];
+        const int ge = ((nk + zq) * E + (nj + py)) * E + (ni + x0);
+        q0 = A.nl.ptr[ge];
+        q1 = A.nl.ptr[ge + 1];
+        q2 = has1 ? A.nl.ptr[ge + 2] : q1;
+    };
+    auto gather = [&](int q0, int q1, int q2, T &a0, T &a1) {
+        a0 = a1 = 0;
+        for (int e = q0; e < q1; ++e) a0 += (T)sv[A.nl.ip[e]] * A.nl.u[e];
+        for (int e = q1; e < q2; ++e) a1 += (T)sv[A.nl.ip[e]] * A.nl.u[e];
+    };
     T nx0 = 0, nx1 = 0, np0 = 0, np1 = 0;   // plane z+6 prefetched one step ahead
     if (act && FDN < M) {
         nx0 = load_in(FDN, x0, np0);
         if (has1) nx1 = load_in(FDN, x0 + 1, np1);
     }
+    T tpn0, tpn1, ven0, ven1, nln0, nln1;   // next plane's pointwise inputs
+    int qn0, qn1, qn2;
+    pointwise(0, tpn0, tpn1, ven0, ven1, qn0, qn1, qn2);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
+    gather(qn0, qn1, qn2, nln0, nln1);
 
     for (int zb = 0; zb < M; zb += WIN) {
 #pragma unroll
@@ -196,33 +254,14 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
             const int sc = u;                          // slot of plane z
             const int sn = (u + FDN) % WIN;            // slot of plane z+6
             w0[sn] = nx0; w1[sn] = nx1; p0[sn] = np0; p1[sn] = np1;
+            const T tp0 = tpn0, tp1 = tpn1, ve0 = ven0, ve1 = ven1, nl0 = nln0, nl1 = nln1;
             nx0 = nx1 = np0 = np1 = 0;
             if (act && z + FDN + 1 < M) {
                 nx0 = load_in(z + FDN + 1, x0, np0);
                 if (has1) nx1 = load_in(z + FDN + 1, x0 + 1, np1);
             }
-            // pointwise inputs of this plane
+            pointwise(z + 1, tpn0, tpn1, ven0, ven1, qn0, qn1, qn2);
             const size_t ozp = base + (size_t)z * M2 + col;
-            T tp0 = 0, tp1 = 0;
-            if (MODE == MODE_CHEB && !A.first && act) {
-                tp0 = Y[ozp];
-                if (has1) tp1 = Y[ozp + 1];
-            }
-            const int gz = wrapn(nk - nloc + z, n);
-            T ve0 = 0, ve1 = 0;
-            if (act) {
-                const size_t vo = ((size_t)gz * n) * n + gyo;
-                ve0 = A.veff[vo + gx0];
-                if (has1) ve1 = A.veff[vo + gx1];
-            }
-            T nl0 = 0, nl1 = 0;
-            if (act) {
-                const int ge = ((nk + z) * E + (nj + py)) * E + (ni + x0);
-                const int e0 = A.nl.ptr[ge], e1 = A.nl.ptr[ge + 1];
-                const int e2 = has1 ? A.nl.ptr[ge + 2] : e1;
-                for (int e = e0; e < e1; ++e) nl0 += (T)sIn[A.nl.ip[e]] * A.nl.u[e];
-                for (int e = e1; e < e2; ++e) nl1 += (T)sIn[A.nl.ip[e]] * A.nl.u[e];
-            }
             // stage the current plane
             T *pl = lds + (z & 1) * G.plane;
             const T c0 = w0[sc], c1 = w1[sc];
@@ -230,7 +269,7 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
                 pl[(py + FDN) * PP + x0 + FDN] = c0;
                 if (has1) pl[(py + FDN) * PP + x0 + FDN + 1] = c1;
             }
-            __syncthreads();
+            lds_barrier();
             if (act) {
                 const T *row = pl + (py + FDN) * PP + x0;   // padded x0 .. x0+13
                 const P2 r0 = *reinterpret_cast<const P2 *>(row + 0);
@@ -278,6 +317,17 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
                         acc[0] += (double)o1 * (double)o1;
                         acc[1] += (double)o1 * (double)c1;
                     }
+                    if (A.D) {
+                        T *Dp = A.D + ozp;
+                        if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
+                            const bool cz = z == nloc && py == nloc;
+                            Dp[0] = o0 * dk + ((cz && x0 == nloc) ? d0 : (T)0);
+                            if (has1) Dp[1] = o1 * dk + ((cz && x0 + 1 == nloc) ? d0 : (T)0);
+                        } else {
+                            Dp[0] += o0 * dk;
+                            if (has1) Dp[1] += o1 * dk;
+                        }
+                    }
                     if (z == nloc && py == nloc) {
                         if (x0 == nloc) sh_center = (double)o0;
                         else if (x0 + 1 == nloc) sh_center = (double)o1;
@@ -301,6 +351,7 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
                     }
                 }
             }
+            gather(qn0, qn1, qn2, nln0, nln1);   // projector apply of plane z+1
         }
     }
     if (MODE == MODE_HSUB) return;
@@ -393,6 +444,117 @@ __global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
                 const double r = wave_sum(acc[i]);
                 if (lane == 0) sb[A.ip_base[J] + i] = r * A.inv_denom[16 * t + i];
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- nonlocal forces
+constexpr int FSLOT = 32;   // images whose rc-box holds one node centre
+
+template <typename T>
+struct FArgs {
+    Geo g;
+    const T *D;                   // [B][M^3] density-matrix columns
+    const int *list, *list_cnt;
+    const int *img_type, *img_nn, *img_atom, *nlocc, *nproj;
+    const long *udv_off;
+    const T *udv;
+    const double *inv_denom;      // [type][16]
+    double grad[FDN + 1];         // coeff_grad
+    double *contrib;              // [B][FSLOT][4]: atom, and sum_lm s_lm,a U_lm(centre) for a = x,y,z
+    int *ncontrib, *overflow;
+};
+
+// For each image whose rc-box holds node b's centre:
+//   s_lm,a = <U_lm, d_a D>/Denom over the rc-box cut to the node cube, with
+//   the FD gradient of D taken with zero values outside the cube, and the
+//   image's term of [V_nl,J d_a D](centre) = sum_lm s_lm,a U_lm(centre)
+// (forces.cpp:463-529 with VnlocsubTimesVec_J, nonlocal.cpp:522-724).
+// One wave per image; slots follow the order of the node's image list.
+template <typename T, int MAXP>
+__global__ void __launch_bounds__(256) k_force_contract(FArgs<T> A)
+{
+    const Geo &G = A.g;
+    const int b = blockIdx.x;
+    const int node = G.node_list[b];
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
+    const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int cnt = A.list_cnt[b];
+    __shared__ int slot_of[LCAP];
+    if (threadIdx.x == 0) {
+        int r = 0;
+        for (int li = 0; li < cnt; ++li) {
+            const int J = A.list[(size_t)b * LCAP + li];
+            const int nc = A.nlocc[A.img_type[J]];
+            const bool in = abs(A.img_nn[3 * J] - ni) <= nc && abs(A.img_nn[3 * J + 1] - nj) <= nc &&
+                            abs(A.img_nn[3 * J + 2] - nk) <= nc;
+            slot_of[li] = (in && r < FSLOT) ? r : -1;
+            if (in) ++r;
+        }
+        if (r > FSLOT) atomicOr(A.overflow, 1);
+        A.ncontrib[b] = r < FSLOT ? r : FSLOT;
+    }
+    __syncthreads();
+    const T *__restrict__ D = A.D + (size_t)b * G.M3;
+    auto Dat = [&](int x, int y, int z) -> double {
+        return (x < 0 || y < 0 || z < 0 || x >= M || y >= M || z >= M) ? 0.0
+                                                                        : (double)D[(size_t)z * M2 + y * M + x];
+    };
+    for (int li = wave; li < cnt; li += nwave) {
+        const int slot = slot_of[li];
+        if (slot < 0) continue;
+        const int J = A.list[(size_t)b * LCAP + li];
+        const int t = A.img_type[J];
+        const int nc = A.nlocc[t], np = A.nproj[t], w = 2 * nc + 1, w2 = w * w, w3 = w2 * w;
+        const int cx = A.img_nn[3 * J], cy = A.img_nn[3 * J + 1], cz = A.img_nn[3 * J + 2];
+        const int lox = max(cx - nc, ni - nloc), hix = min(cx + nc, ni + nloc);
+        const int loy = max(cy - nc, nj - nloc), hiy = min(cy + nc, nj + nloc);
+        const int loz = max(cz - nc, nk - nloc), hiz = min(cz + nc, nk + nloc);
+        const int bx = hix - lox + 1, area = bx * (hiy - loy + 1);
+        const T *U = A.udv + A.udv_off[J];
+        double acc[MAXP][3];
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i) acc[i][0] = acc[i][1] = acc[i][2] = 0.0;
+        for (int a = lane; a < area; a += 64) {
+            const int iy = a / bx, ix = a - iy * bx;
+            const int gx = lox + ix, gy = loy + iy;
+            const int x = gx - ni + nloc, y = gy - nj + nloc;
+            const int ridx = (gy - cy + nc) * w + (gx - cx + nc);
+            for (int gz = loz; gz <= hiz; ++gz) {
+                const int z = gz - nk + nloc;
+                double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+                for (int s = 1; s <= FDN; ++s) {
+                    g0 += (-Dat(x - s, y, z) + Dat(x + s, y, z)) * A.grad[s];
+                    g1 += (-Dat(x, y - s, z) + Dat(x, y + s, z)) * A.grad[s];
+                    g2 += (-Dat(x, y, z - s) + Dat(x, y, z + s)) * A.grad[s];
+                }
+                const int q = (gz - cz + nc) * w2 + ridx;
+#pragma unroll
+                for (int i = 0; i < MAXP; ++i)
+                    if (i < np) {
+                        const double uq = (double)U[(size_t)i * w3 + q];
+                        acc[i][0] += g0 * uq;
+                        acc[i][1] += g1 * uq;
+                        acc[i][2] += g2 * uq;
+                    }
+            }
+        }
+        double f[3] = {0.0, 0.0, 0.0};
+        const int qc = (nk - cz + nc) * w2 + (nj - cy + nc) * w + (ni - cx + nc);
+#pragma unroll
+        for (int i = 0; i < MAXP; ++i)
+            if (i < np) {
+                const double uc = (double)U[(size_t)i * w3 + qc];
+                const double id = A.inv_denom[16 * t + i];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) f[d] += wave_sum(acc[i][d]) * id * uc;
+            }
+        if (lane == 0) {
+            double *o = A.contrib + ((size_t)b * FSLOT + slot) * 4;
+            o[0] = (double)A.img_atom[J];
+            o[1] = f[0]; o[2] = f[1]; o[3] = f[2];
         }
     }
 }

@@ -93,6 +93,24 @@ def test_density_matches_oracle(cases, name):
     assert H.rel(out2["rho"], ref2["rho"]) < FP64_TOL
 
 
+@pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])
+def test_nonlocal_forces_match_oracle(cases, name):
+    S, osys, veff = cases[name]
+    say("forces", name)
+    ref = osys.density(veff, scf_iter=1, nthreads=0)
+    fref = osys.forces_nonlocal(veff, ref["chi"], ref["zee"], ref["Ci"], nthreads=0)
+    sq = SpectralQuadrature(S, precision="fp64")
+    sq.density(veff, scf_iter=1)
+    f = sq.ForcesNonLocal()
+    assert np.max(np.abs(f - fref)) <= 1e-10 * np.max(np.abs(fref)) + 1e-14
+    # mixed path: FP32 recurrence and accumulator
+    sqm = SpectralQuadrature(S, precision="mixed")
+    sqm.density(veff, scf_iter=1)
+    fm = sqm.ForcesNonLocal()
+    say("mixed force err", np.max(np.abs(fm - fref)))
+    assert np.max(np.abs(fm - fref)) < 1e-4
+
+
 def test_density_batching_is_invisible(cases):
     S, osys, veff = cases["small_al"]
     say("batching")

@@ -72,6 +72,13 @@ def load() -> C.CDLL:
                                        c_int_p]),
         "sqo_moments_nodes": (C.c_int, [P, c_dbl_p, c_int_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p,
                                         C.c_int]),
+        "sqo_moments_fixed": (C.c_int, [P, c_dbl_p, C.c_int, C.c_double, C.c_double, c_dbl_p]),
+        "sqo_density_fixed": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double, C.c_double,
+                                        c_dbl_p, c_dbl_p, c_dbl_p, C.c_int]),
+        "sqo_adjoint_veff": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, c_dbl_p, C.c_int,
+                                       c_dbl_p, C.c_int]),
+        "sqo_tangent_veff": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, c_dbl_p, C.c_int,
+                                       c_dbl_p, C.c_int]),
         "sqo_ne_constraint": (C.c_double, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double]),
         "sqo_brent": (C.c_int, [P, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double, C.c_double,
                                 C.c_int, C.c_double, c_dbl_p, c_int_p]),
@@ -214,6 +221,39 @@ class OracleSystem:
         return {"status": st, "rho": rho, "lambda_f": lf.value, "ebs": ebs.value, "eent": eent.value,
                 "chi": chi, "zee": zee, "mu": mu.reshape(nn, K), "Ci": ci.reshape(nn, K), "iters": its,
                 "fermi_iters": fit.value}
+
+    def density_fixed(self, veff, chi, zee, lf_lo, lf_hi, ncharge=None, nthreads=0) -> dict:
+        nn, K = self.n ** 3, self.npl + 1
+        if ncharge is None:
+            ncharge = -float(sum(self.S.Z[t] * self.S.natm[t] for t in range(self.S.n_typ)))
+        lf = C.c_double()
+        rho = np.zeros(nn); mu = np.zeros(nn * K)
+        st = self._lib.sqo_density_fixed(C.byref(self.S), _d(np.ascontiguousarray(veff)),
+                                         _d(np.ascontiguousarray(chi)), _d(np.ascontiguousarray(zee)),
+                                         float(ncharge), lf_lo, lf_hi, C.byref(lf), _d(rho), _d(mu), int(nthreads))
+        if st < 0:
+            raise RuntimeError(f"sqo_density_fixed: {st}")
+        return {"rho": rho, "lambda_f": lf.value, "mu": mu.reshape(nn, K)}
+
+    def adjoint_veff(self, veff, chi, zee, mu, lambda_f, w, fixed_n=1, nthreads=0) -> np.ndarray:
+        out = np.zeros(self.n ** 3)
+        st = self._lib.sqo_adjoint_veff(C.byref(self.S), _d(np.ascontiguousarray(veff)),
+                                        _d(np.ascontiguousarray(chi)), _d(np.ascontiguousarray(zee)),
+                                        _d(np.ascontiguousarray(mu)), lambda_f, _d(np.ascontiguousarray(w)),
+                                        int(fixed_n), _d(out), int(nthreads))
+        if st < 0:
+            raise RuntimeError(f"sqo_adjoint_veff: {st}")
+        return out
+
+    def tangent_veff(self, veff, chi, zee, mu, lambda_f, dv, fixed_n=1, nthreads=0) -> np.ndarray:
+        out = np.zeros(self.n ** 3)
+        st = self._lib.sqo_tangent_veff(C.byref(self.S), _d(np.ascontiguousarray(veff)),
+                                        _d(np.ascontiguousarray(chi)), _d(np.ascontiguousarray(zee)),
+                                        _d(np.ascontiguousarray(mu)), lambda_f, _d(np.ascontiguousarray(dv)),
+                                        int(fixed_n), _d(out), int(nthreads))
+        if st < 0:
+            raise RuntimeError(f"sqo_tangent_veff: {st}")
+        return out
 
     def forces_nonlocal(self, veff, chi, zee, Ci, nodes=None, nthreads=0) -> np.ndarray:
         na = self.S.n_atm

@@ -752,6 +752,291 @@ int sqo_node_moments(const sqo_sys *S, const double *veff, int node,
     return st;
 }
 
+/* ------------------------------------------------------------------ */
+/* Derivatives with respect to Veff (chi, zee frozen)                  */
+/* ------------------------------------------------------------------ */
+int sqo_moments_fixed(const sqo_sys *S, const double *veff, int node, double ch, double ze, double *mu)
+{   /* sq.cpp:568-655 with given chi, zee */
+    int nloc = S->nloc, M = 2 * nloc + 1, npl = S->npl;
+    long N = (long)M * M * M, c = ((long)nloc * M + nloc) * M + nloc;
+    double *t0 = (double *)calloc((size_t)N, sizeof(double));
+    double *t1 = (double *)malloc((size_t)N * sizeof(double));
+    double *t2 = (double *)malloc((size_t)N * sizeof(double));
+    double *Hv = (double *)malloc((size_t)N * sizeof(double));
+    double h3 = pow(S->h, 3);
+    t0[c] = 1.0;
+    int st = sqo_hsub(S, veff, node, t0, Hv);
+    for (long i = 0; i < N; i++) t1[i] = (Hv[i] - ch * t0[i]) / ze;
+    double htemp = t1[c];
+    int q4 = npl / 4, q2 = npl / 2;
+    mu[0] = t0[c] / h3;
+    mu[1] = t1[c] / h3;
+    for (int nq = 2; nq <= q2; nq++) {
+        double d1 = 0.0, d2 = 0.0;
+        st |= sqo_hsub(S, veff, node, t1, Hv);
+        for (long i = 0; i < N; i++) {
+            t2[i] = (2 * (Hv[i] - ch * t1[i]) / ze) - t0[i];
+            d1 += t2[i] * t2[i];
+            d2 += t2[i] * t1[i];
+            t0[i] = t1[i];
+            t1[i] = t2[i];
+        }
+        mu[nq] = t2[c] / h3;
+        if (nq > q4) {
+            mu[2 * nq] = (2 * d1 - 1) / h3;
+            mu[2 * nq - 1] = (2 * d2 - htemp) / h3;
+        }
+    }
+    free(t0); free(t1); free(t2); free(Hv);
+    return st;
+}
+
+int sqo_density_fixed(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                      double ncharge, double lf_lo, double lf_hi, double *lambda_f, double *rho,
+                      double *mu, int nthreads)
+{
+    int n = S->n, npl = S->npl, nn = n * n * n, st = SQO_OK;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int nd = 0; nd < nn; nd++) {
+        int s = sqo_moments_fixed(S, veff, nd, chi[nd], zee[nd], mu + (long)nd * (npl + 1));
+        if (s < 0) {
+#pragma omp critical
+            st = s;
+        }
+    }
+    if (st < 0) return st;
+    double lf;
+    int it;
+    st = sqo_brent(S, nn, chi, zee, mu, ncharge, lf_lo, lf_hi, 1, 0.0, &lf, &it);
+    if (st < 0) return st;
+    *lambda_f = lf;
+    return sqo_rho_energy(S, nn, chi, zee, mu, lf, rho, NULL, NULL, NULL);
+}
+
+/* apply A = (H - chi)/zee */
+static int apply_A(const sqo_sys *S, const double *veff, int node, double ch, double ze,
+                   const double *v, double *out, double *tmp, long N)
+{
+    int st = sqo_hsub(S, veff, node, v, tmp);
+    for (long i = 0; i < N; i++) out[i] = (tmp[i] - ch * v[i]) / ze;
+    return st;
+}
+
+/* per-node rho_pc, its lambda-derivative and the Fermi coefficients */
+static void node_fermi(const sqo_sys *S, const double *m, double ch, double ze, double lf,
+                       double *rho_pc, double *drho_dl, double *C, double *Cd)
+{
+    int npl = S->npl;
+    double lfp = (lf - ch) / ze, bet = S->bet * ze;
+    sqo_cheb_coeff(npl, lfp, bet, C, 0);
+    sqo_cheb_coeff(npl, lfp, bet, Cd, 3);
+    double r = 0.0, d = 0.0;
+    for (int j = 0; j <= npl; j++) { r += 2 * m[j] * C[j]; d += 2 * m[j] * Cd[j]; }
+    *rho_pc = r;
+    *drho_dl = d * S->bet;
+}
+
+static void map_cube_to_grid(const sqo_sys *S, int node, long *g)
+{
+    int n = S->n, nloc = S->nloc, M = 2 * nloc + 1;
+    int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    long p = 0;
+    for (int z = 0; z < M; z++)
+        for (int y = 0; y < M; y++)
+            for (int x = 0; x < M; x++, p++)
+                g[p] = ((long)wrapi(nk - nloc + z, n) * n + wrapi(nj - nloc + y, n)) * n + wrapi(ni - nloc + x, n);
+}
+
+int sqo_adjoint_veff(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                     const double *mu, double lambda_f, const double *w, int fixed_n,
+                     double *dLdV, int nthreads)
+{
+    int n = S->n, npl = S->npl, nn = n * n * n, nloc = S->nloc, M = 2 * nloc + 1;
+    long N = (long)M * M * M, c = ((long)nloc * M + nloc) * M + nloc;
+    int K2 = npl / 2, q4 = npl / 4;
+    double h3 = pow(S->h, 3);
+    double *sgn = (double *)malloc((size_t)nn * sizeof(double));
+    double *dl = (double *)malloc((size_t)nn * sizeof(double));
+    double *C = (double *)malloc((size_t)(npl + 1) * sizeof(double));
+    double *Cd = (double *)malloc((size_t)(npl + 1) * sizeof(double));
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nn; i++) {
+        double r;
+        node_fermi(S, mu + (long)i * (npl + 1), chi[i], zee[i], lambda_f, &r, &dl[i], C, Cd);
+        sgn[i] = r < 0 ? -1.0 : 1.0;
+        num += w[i] * sgn[i] * dl[i];
+        den += sgn[i] * dl[i];
+    }
+    double kappa = fixed_n ? num / den : 0.0;
+    free(C); free(Cd);
+    int nth = 1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    nth = omp_get_max_threads();
+#endif
+    double *acc = (double *)calloc((size_t)nth * nn, sizeof(double));
+    int st = SQO_OK;
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *T = (double *)malloc((size_t)(K2 + 1) * N * sizeof(double));
+        double *tb0 = (double *)malloc((size_t)N * sizeof(double));   /* tbar_{k}   */
+        double *tb1 = (double *)calloc((size_t)N, sizeof(double));    /* tbar_{k+1} */
+        double *tb2 = (double *)calloc((size_t)N, sizeof(double));    /* tbar_{k+2} */
+        double *G = (double *)malloc((size_t)N * sizeof(double));
+        double *tmp = (double *)malloc((size_t)N * sizeof(double));
+        double *tmp2 = (double *)malloc((size_t)N * sizeof(double));
+        double *Cn = (double *)malloc((size_t)(npl + 1) * sizeof(double));
+        double *Cdn = (double *)malloc((size_t)(npl + 1) * sizeof(double));
+        double *be = (double *)malloc((size_t)(npl + 2) * sizeof(double));
+        long *gidx = (long *)malloc((size_t)N * sizeof(long));
+        double *my = acc + (size_t)tid * nn;
+#pragma omp for schedule(static)
+        for (int i = 0; i < nn; i++) {
+            const double *m = mu + (long)i * (npl + 1);
+            double r, d;
+            node_fermi(S, m, chi[i], zee[i], lambda_f, &r, &d, Cn, Cdn);
+            double wt = (w[i] - kappa) * sgn[i];
+            for (int j = 0; j <= npl; j++) be[j] = 2 * Cn[j] * wt;
+            /* forward vectors t_0..t_K2 */
+            double *t = T;
+            for (long p = 0; p < N; p++) t[p] = 0.0;
+            t[c] = 1.0;
+            int s = apply_A(S, veff, i, chi[i], zee[i], t, T + N, tmp, N);
+            for (int k = 1; k < K2; k++) {
+                s |= apply_A(S, veff, i, chi[i], zee[i], T + k * N, tmp2, tmp, N);
+                for (long p = 0; p < N; p++) T[(k + 1) * N + p] = 2 * tmp2[p] - T[(k - 1) * N + p];
+            }
+            /* backward (Clenshaw-type) recurrence of the adjoint vectors */
+            double sum_odd = 0.0;
+            for (int k = q4 + 1; k <= K2; k++) sum_odd += be[2 * k - 1];
+            for (long p = 0; p < N; p++) { tb1[p] = 0.0; tb2[p] = 0.0; G[p] = 0.0; }
+            for (int k = K2; k >= 1; k--) {
+                /* direct gradient g_k */
+                const double *tk = T + (long)k * N;
+                for (long p = 0; p < N; p++) tb0[p] = 0.0;
+                tb0[c] += be[k] / h3;
+                if (k > q4) {
+                    const double *tkm = T + (long)(k - 1) * N;
+                    for (long p = 0; p < N; p++) tb0[p] += (4 * be[2 * k] * tk[p] + 2 * be[2 * k - 1] * tkm[p]) / h3;
+                }
+                if (k + 1 > q4 && k + 1 <= K2) {
+                    const double *tkp = T + (long)(k + 1) * N;
+                    for (long p = 0; p < N; p++) tb0[p] += 2 * be[2 * k + 1] * tkp[p] / h3;
+                }
+                if (k == 1) tb0[c] -= sum_odd / h3;
+                if (k < K2) {
+                    s |= apply_A(S, veff, i, chi[i], zee[i], tb1, tmp2, tmp, N);
+                    for (long p = 0; p < N; p++) tb0[p] += 2 * tmp2[p] - tb2[p];
+                }
+                /* G += 2 tbar_{k+1} t_k (k >= 1); the k = 0 term uses tbar_1 */
+                if (k < K2)
+                    for (long p = 0; p < N; p++) G[p] += 2 * tb1[p] * tk[p];
+                double *r2 = tb2; tb2 = tb1; tb1 = tb0; tb0 = r2;
+            }
+            /* tb1 now holds tbar_1; t_1 = A t_0 contributes tbar_1 t_0^T */
+            G[c] += tb1[c] * 1.0;
+            map_cube_to_grid(S, i, gidx);
+            for (long p = 0; p < N; p++) my[gidx[p]] += G[p] / zee[i];
+            if (s < 0) {
+#pragma omp critical
+                st = s;
+            }
+        }
+        free(T); free(tb0); free(tb1); free(tb2); free(G); free(tmp); free(tmp2);
+        free(Cn); free(Cdn); free(be); free(gidx);
+    }
+    for (int i = 0; i < nn; i++) {
+        double s = 0.0;
+        for (int t = 0; t < nth; t++) s += acc[(size_t)t * nn + i];
+        dLdV[i] = s;
+    }
+    free(acc); free(sgn); free(dl);
+    return st;
+}
+
+int sqo_tangent_veff(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                     const double *mu, double lambda_f, const double *dV, int fixed_n,
+                     double *drho, int nthreads)
+{
+    int n = S->n, npl = S->npl, nn = n * n * n, nloc = S->nloc, M = 2 * nloc + 1;
+    long N = (long)M * M * M, c = ((long)nloc * M + nloc) * M + nloc;
+    int K2 = npl / 2, q4 = npl / 4;
+    double h3 = pow(S->h, 3);
+    double *dl = (double *)malloc((size_t)nn * sizeof(double));
+    double *sgn = (double *)malloc((size_t)nn * sizeof(double));
+    int st = SQO_OK;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        double *t0 = (double *)calloc((size_t)N, sizeof(double)), *t1 = (double *)malloc((size_t)N * 8);
+        double *d0 = (double *)calloc((size_t)N, sizeof(double)), *d1 = (double *)malloc((size_t)N * 8);
+        double *t2 = (double *)malloc((size_t)N * 8), *dd2 = (double *)malloc((size_t)N * 8);
+        double *a = (double *)malloc((size_t)N * 8), *tmp = (double *)malloc((size_t)N * 8);
+        double *dA = (double *)malloc((size_t)N * 8), *dmu = (double *)malloc((size_t)(npl + 1) * 8);
+        double *Cn = (double *)malloc((size_t)(npl + 1) * 8), *Cdn = (double *)malloc((size_t)(npl + 1) * 8);
+        long *gidx = (long *)malloc((size_t)N * sizeof(long));
+#pragma omp for schedule(static)
+        for (int i = 0; i < nn; i++) {
+            map_cube_to_grid(S, i, gidx);
+            for (long p = 0; p < N; p++) { dA[p] = dV[gidx[p]] / zee[i]; t0[p] = 0.0; d0[p] = 0.0; }
+            t0[c] = 1.0;
+            int s = apply_A(S, veff, i, chi[i], zee[i], t0, t1, tmp, N);
+            for (long p = 0; p < N; p++) d1[p] = dA[p] * t0[p];
+            double dh = d1[c];
+            for (int j = 0; j <= npl; j++) dmu[j] = 0.0;
+            dmu[1] = d1[c] / h3;
+            for (int k = 2; k <= K2; k++) {
+                s |= apply_A(S, veff, i, chi[i], zee[i], t1, a, tmp, N);
+                for (long p = 0; p < N; p++) t2[p] = 2 * a[p] - t0[p];
+                s |= apply_A(S, veff, i, chi[i], zee[i], d1, a, tmp, N);
+                double s1 = 0.0, s2 = 0.0;
+                for (long p = 0; p < N; p++) {
+                    dd2[p] = 2 * (dA[p] * t1[p] + a[p]) - d0[p];
+                    s1 += t2[p] * dd2[p];
+                    s2 += dd2[p] * t1[p] + t2[p] * d1[p];
+                }
+                dmu[k] = dd2[c] / h3;
+                if (k > q4) {
+                    dmu[2 * k] = 4 * s1 / h3;
+                    dmu[2 * k - 1] = (2 * s2 - dh) / h3;
+                }
+                double *r = t0; t0 = t1; t1 = t2; t2 = r;
+                r = d0; d0 = d1; d1 = dd2; dd2 = r;
+            }
+            double rp, dlr;
+            node_fermi(S, mu + (long)i * (npl + 1), chi[i], zee[i], lambda_f, &rp, &dlr, Cn, Cdn);
+            double sg = rp < 0 ? -1.0 : 1.0, v = 0.0;
+            for (int j = 0; j <= npl; j++) v += 2 * Cn[j] * dmu[j];
+            drho[i] = sg * v;
+            dl[i] = sg * dlr;
+            sgn[i] = sg;
+            if (s < 0) {
+#pragma omp critical
+                st = s;
+            }
+        }
+        free(t0); free(t1); free(d0); free(d1); free(t2); free(dd2); free(a); free(tmp); free(dA);
+        free(dmu); free(Cn); free(Cdn); free(gidx);
+    }
+    if (fixed_n) {
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < nn; i++) { num += drho[i]; den += dl[i]; }
+        double dlam = -num / den;
+        for (int i = 0; i < nn; i++) drho[i] += dl[i] * dlam;
+    }
+    free(dl); free(sgn);
+    return st;
+}
+
 int sqo_moments_nodes(const sqo_sys *S, const double *veff, const int *nodes, int count,
                       double *chi, double *zee, double *mu, int *iters, int nthreads)
 {

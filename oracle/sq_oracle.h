@@ -147,6 +147,28 @@ int sqo_rho_energy(const sqo_sys *S, int nn, const double *chi, const double *ze
                    const double *mu, double lambda_f, double *rho, double *ebs, double *eent,
                    double *Ci);
 
+/* ---- derivatives with respect to Veff (no counterpart in the reference) ----
+ * chi/zee are frozen (the Lanczos bounds are not differentiated).  The
+ * moments follow sq.cpp:568-655 exactly (centres up to npl/2, the doubling
+ * dot products above npl/4), and these functions differentiate that
+ * computation.  fixed_n = 1 adds the Fermi-level shift that keeps the
+ * electron count fixed. */
+/* Chebyshev moments of one node with given chi, zee (sq.cpp:568-655). */
+int sqo_moments_fixed(const sqo_sys *S, const double *veff, int node, double chi, double zee,
+                      double *mu);
+/* density step with frozen chi/zee for all nodes: moments, Brent, rho */
+int sqo_density_fixed(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                      double ncharge, double lf_lo, double lf_hi, double *lambda_f, double *rho,
+                      double *mu, int nthreads);
+/* dL/dVeff for L = sum_i w[i] rho[i] (reverse mode through the recurrence) */
+int sqo_adjoint_veff(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                     const double *mu, double lambda_f, const double *w, int fixed_n,
+                     double *dLdV, int nthreads);
+/* drho along dV (forward mode through the recurrence) */
+int sqo_tangent_veff(const sqo_sys *S, const double *veff, const double *chi, const double *zee,
+                     const double *mu, double lambda_f, const double *dV, int fixed_n,
+                     double *drho, int nthreads);
+
 /* Nonlocal forces (forces.cpp:318-578), uses chi, zee, Ci of the last SQ call.
  * f has 3*n_atm entries laid out fx[n_atm], fy[n_atm], fz[n_atm].
  * nodes==NULL -> all n^3 nodes; otherwise only the listed nodes contribute. */

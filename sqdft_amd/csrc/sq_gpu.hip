@@ -118,6 +118,37 @@ Geo make_geo(const sqg_ctx *c, const int *dnodes, const int *ddone)
 
 int zmarch_threads(int M) { return (((M + 1) / 2 * M) + 63) / 64 * 64; }
 
+// Visiting order of the FD nodes.  Nodes are enumerated in 4x4x2 bricks so
+// that cubes processed at the same time overlap, then each batch of B
+// consecutive nodes is dealt out so that workgroup i (dispatched to XCD
+// i mod 8) takes the i/8-th node of the (i mod 8)-th contiguous eighth of the
+// batch: every XCD walks a compact region and its L2 serves the Veff and
+// projector data that neighbouring cubes share.
+std::vector<int> node_order(int n, int B)
+{
+    const int bx = 4, by = 4, bz = 2, NX = 8;
+    std::vector<int> seq;
+    seq.reserve((size_t)n * n * n);
+    for (int z0 = 0; z0 < n; z0 += bz)
+        for (int y0 = 0; y0 < n; y0 += by)
+            for (int x0 = 0; x0 < n; x0 += bx)
+                for (int z = z0; z < std::min(z0 + bz, n); ++z)
+                    for (int y = y0; y < std::min(y0 + by, n); ++y)
+                        for (int x = x0; x < std::min(x0 + bx, n); ++x) seq.push_back((z * n + y) * n + x);
+    std::vector<int> out(seq.size());
+    for (size_t s = 0; s < seq.size(); s += B) {
+        const size_t len = std::min(seq.size() - s, (size_t)B);
+        const size_t ch = (len + NX - 1) / NX;
+        size_t o = s;
+        for (size_t i = 0; i < ch; ++i)
+            for (int k = 0; k < NX; ++k) {
+                const size_t src = k * ch + i;
+                if (src < len) out[o++] = seq[s + src];
+            }
+    }
+    return out;
+}
+
 // Allocates per-batch buffers for up to B nodes.
 int ensure_batch(sqg_ctx *c, int B)
 {
@@ -416,9 +447,9 @@ int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int
     return SQG_OK;
 }
 
-int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes)
+int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes, int cap = 0)
 {
-    const int cap = choose_batch(c, count);
+    if (cap <= 0) cap = choose_batch(c, count);
     if (c->p.precision == SQG_MIXED) return moments_for_nodes<float>(c, nodes, count, passes, cap);
     return moments_for_nodes<double>(c, nodes, count, passes, cap);
 }
@@ -814,10 +845,10 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     for (int w = 0; w < 2; ++w) c->ev[w].clear();
     c->pool_used = 0;
     HIPCHK(hipEventRecord(t0, c->st));
-    std::vector<int> all(c->nn);
-    for (int i = 0; i < c->nn; ++i) all[i] = i;
+    const int cap = choose_batch(c, c->nn);
+    const std::vector<int> all = node_order(c->p.n, cap);
     long passes = 0;
-    st = moments_dispatch(c, all.data(), c->nn, &passes);
+    st = moments_dispatch(c, all.data(), c->nn, &passes, cap);
     if (st) return st;
     HIPCHK(hipEventRecord(t1, c->st));
     // global spectrum bounds (sq.cpp:676-713)
@@ -969,8 +1000,7 @@ int sqg_forces_nonlocal(sqg_ctx *c, double *f)
     if (c->n_ip == 0) return SQG_OK;
     std::vector<double> rec;   // node, atom, fx, fy, fz per contributing image
     const int cap = choose_batch(c, c->nn, 3);
-    std::vector<int> nodes(c->nn);
-    for (int i = 0; i < c->nn; ++i) nodes[i] = i;
+    const std::vector<int> nodes = node_order(c->p.n, cap);
     int done_n = 0;
     while (done_n < c->nn) {
         const int B = std::min(cap, c->nn - done_n);
@@ -984,6 +1014,16 @@ int sqg_forces_nonlocal(sqg_ctx *c, double *f)
     }
     // sum in node order; within a node, all images of one atom form one
     // [V_nl,J d D](centre) term as in forces.cpp:509-520
+    {
+        const size_t nr0 = rec.size() / 5;
+        std::vector<size_t> idx(nr0);
+        for (size_t q = 0; q < nr0; ++q) idx[q] = q;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b2) { return rec[5 * a] < rec[5 * b2]; });
+        std::vector<double> sorted(rec.size());
+        for (size_t q = 0; q < nr0; ++q)
+            for (int d = 0; d < 5; ++d) sorted[5 * q + d] = rec[5 * idx[q] + d];
+        rec.swap(sorted);
+    }
     const double h3 = std::pow(c->p.h, 3);
     size_t i = 0;
     const size_t nr = rec.size() / 5;

@@ -130,6 +130,28 @@ int sqg_hsub_apply_batch(sqg_ctx *ctx, const double *veff, const int *nodes, int
 int sqg_last_kernel_timing(sqg_ctx *ctx, int which, long *launches, double *total_ms,
                            double *alg_bytes);
 
+/* ---- Derivatives with respect to Veff (no counterpart in the reference) ----
+ * All three use the state of the last density call (sqg_density or
+ * sqg_density_fixed): Veff, chi, zee, moments, lambda_f and Ci.  chi and zee
+ * stay frozen (the Lanczos bounds are not differentiated); the moments are
+ * differentiated exactly as src/sq.cpp:568-655 computes them (direct centres
+ * up to npl/2, the doubling dot products above npl/4). */
+
+/* The SQ density step with the last step's chi, zee kept: moments, Brent
+ * root in [lf_lo, lf_hi] (the scf_iter = 1 bracket of src/sq.cpp:1289-1404),
+ * rho.  The finite-difference partner of the two calls below. */
+int sqg_density_fixed(sqg_ctx *ctx, const double *veff, double ncharge, double lf_lo, double lf_hi,
+                      double *lambda_f, double *rho);
+
+/* dL/dVeff (n^3 out) for L = sum_i w[i] rho[i] (w: n^3).  Reverse mode
+ * through the Chebyshev recurrence, recomputing vectors from checkpoints
+ * every ckpt_interval steps.  fixed_n = 1 includes the Fermi-level shift that
+ * keeps the electron count fixed. */
+int sqg_adjoint_veff(sqg_ctx *ctx, const double *w, int fixed_n, double *dLdV);
+
+/* drho (n^3 out) along dV (n^3): forward mode through the recurrence. */
+int sqg_tangent_veff(sqg_ctx *ctx, const double *dV, int fixed_n, double *drho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,10 @@ SIGNATURES = {
                                    c_int_p]),
     "sqg_hsub_apply_batch": (C.c_int, [C.c_void_p, c_dbl_p, c_int_p, C.c_int, c_dbl_p, c_dbl_p]),
     "sqg_last_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, c_long_p, c_dbl_p, c_dbl_p]),
+    "sqg_density_fixed": (C.c_int, [C.c_void_p, c_dbl_p, C.c_double, C.c_double, C.c_double, c_dbl_p,
+                                    c_dbl_p]),
+    "sqg_adjoint_veff": (C.c_int, [C.c_void_p, c_dbl_p, C.c_int, c_dbl_p]),
+    "sqg_tangent_veff": (C.c_int, [C.c_void_p, c_dbl_p, C.c_int, c_dbl_p]),
 }
 
 STATUS = {

@@ -9,6 +9,7 @@
 #include "sqdft_gpu.h"
 #include "sqdft_system.h"
 #include "sq_kernels.hpp"
+#include "sq_adjoint_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -75,6 +76,9 @@ struct sqg_ctx {
     // all-node state of the last density step
     DBuf veff, chi_all, zee_all, lmin_all, lmax_all, iters_all, mu_all, W, Ci_all, rho_d, part;
     DBuf costab, ynode, scal;
+    // Veff derivatives: vectors, per-node sign / d rho/d lambda / weight,
+    // n^3 input and output, per-step tangent records
+    DBuf adj, adj_sgn, adj_dl, adj_wt, adj_in, adj_out, adj_rec;
     bool have_state = false;
     double lambda_f = 0.0;
     // timing of the dominant kernels (which: 0 Chebyshev z-march, 1 Lanczos z-march)
@@ -264,28 +268,13 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
     }
 }
 
-// Lanczos + Chebyshev moments for B nodes already in node_list.
-// Results land in chiB/zeeB/lmin/lmax/cnt (per batch) and mu_all (global).
+// Lanczos bounds of the B nodes in node_list (image lists already built).
+// Results land in chiB/zeeB/lmin/lmax/cnt.
 template <typename T>
-int run_batch(sqg_ctx *c, int B, long *passes_out)
+int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
 {
     const int *dn = c->node_list.as<int>();
     int *flags = c->flags.as<int>();
-    HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
-    // image lists
-    {
-        LArgs L{};
-        L.g = make_geo(c, dn, nullptr);
-        L.n_img = c->n_img;
-        L.img_type = c->img_type.as<int>();
-        L.img_nn = c->img_nn.as<int>();
-        L.nlocc = c->nlocc.as<int>();
-        L.nproj = c->nproj.as<int>();
-        L.list = c->list.as<int>();
-        L.list_cnt = c->list_cnt.as<int>();
-        L.overflow = flags + 0;
-        hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
-    }
     const size_t vlen = (size_t)B * c->M3;
     T *X = c->X.as<T>(), *Y = c->Y.as<T>();
     const double rs = (double)sizeof(T);
@@ -344,6 +333,43 @@ int run_batch(sqg_ctx *c, int B, long *passes_out)
         HIPCHK(hipMemcpy(ha.data(), c->aa.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(hb.data(), c->bb.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < 120; ++i) std::fprintf(stderr, "lanczos b0 pass %d a=%.17g b=%.17g\n", i, ha[i], hb[i]);
+    }
+    return SQG_OK;
+}
+
+// Lanczos + Chebyshev moments for B nodes already in node_list.
+// Results land in chiB/zeeB/lmin/lmax/cnt (per batch) and mu_all (global).
+// Without lanczos the bounds of the last density step are reused.
+template <typename T>
+int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
+{
+    const int *dn = c->node_list.as<int>();
+    int *flags = c->flags.as<int>();
+    HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
+    // image lists
+    {
+        LArgs L{};
+        L.g = make_geo(c, dn, nullptr);
+        L.n_img = c->n_img;
+        L.img_type = c->img_type.as<int>();
+        L.img_nn = c->img_nn.as<int>();
+        L.nlocc = c->nlocc.as<int>();
+        L.nproj = c->nproj.as<int>();
+        L.list = c->list.as<int>();
+        L.list_cnt = c->list_cnt.as<int>();
+        L.overflow = flags + 0;
+        hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+    }
+    const size_t vlen = (size_t)B * c->M3;
+    T *X = c->X.as<T>(), *Y = c->Y.as<T>();
+    const double rs = (double)sizeof(T);
+    if (lanczos) {
+        const int st = lanczos_batch<T>(c, B, passes_out);
+        if (st) return st;
+    } else {   // frozen bounds of the last density step
+        hipLaunchKernelGGL(k_gather_bounds, dim3((B + 127) / 128), dim3(128), 0, c->st, B, dn,
+                           c->chi_all.as<double>(), c->zee_all.as<double>(), c->chiB.as<double>(),
+                           c->zeeB.as<double>());
     }
     // ---------------- Chebyshev moments (sq.cpp:564-655)
     const int c_idx = (c->p.nloc * c->M + c->p.nloc) * c->M + c->p.nloc;
@@ -426,7 +452,7 @@ int ensure_node_state(sqg_ctx *c)
 }
 
 template <typename T>
-int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int batch_cap)
+int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int batch_cap, bool lanczos)
 {
     int done_n = 0;
     while (done_n < count) {
@@ -434,24 +460,25 @@ int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int
         int st = ensure_batch(c, B);
         if (st) return st;
         HIPCHK(hipMemcpyAsync(c->node_list.p, nodes + done_n, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->st));
-        st = run_batch<T>(c, B, passes);
+        st = run_batch<T>(c, B, passes, lanczos);
         if (st) return st;
-        hipLaunchKernelGGL(k_scatter_batch, dim3((B + 127) / 128), dim3(128), 0, c->st, B,
-                           c->node_list.as<int>(), c->chiB.as<double>(), c->zeeB.as<double>(),
-                           c->lmin.as<double>(), c->lmax.as<double>(), c->cnt.as<int>(),
-                           c->chi_all.as<double>(), c->zee_all.as<double>(), c->lmin_all.as<double>(),
-                           c->lmax_all.as<double>(), c->iters_all.as<int>());
+        if (lanczos)
+            hipLaunchKernelGGL(k_scatter_batch, dim3((B + 127) / 128), dim3(128), 0, c->st, B,
+                               c->node_list.as<int>(), c->chiB.as<double>(), c->zeeB.as<double>(),
+                               c->lmin.as<double>(), c->lmax.as<double>(), c->cnt.as<int>(),
+                               c->chi_all.as<double>(), c->zee_all.as<double>(), c->lmin_all.as<double>(),
+                               c->lmax_all.as<double>(), c->iters_all.as<int>());
         done_n += B;
     }
     HIPCHK(hipStreamSynchronize(c->st));
     return SQG_OK;
 }
 
-int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes, int cap = 0)
+int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes, int cap = 0, bool lanczos = true)
 {
     if (cap <= 0) cap = choose_batch(c, count);
-    if (c->p.precision == SQG_MIXED) return moments_for_nodes<float>(c, nodes, count, passes, cap);
-    return moments_for_nodes<double>(c, nodes, count, passes, cap);
+    if (c->p.precision == SQG_MIXED) return moments_for_nodes<float>(c, nodes, count, passes, cap, lanczos);
+    return moments_for_nodes<double>(c, nodes, count, passes, cap, lanczos);
 }
 
 // electron-count constraint g(lambda) (sq.cpp:1183-1228); mode 1 gives d/dlambda / bet
@@ -615,6 +642,36 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
         }
     return SQG_OK;
 }
+
+// rho, Ebs, Eent and the Fermi coefficients Ci at lambda_f from W, chi, zee
+// (sq.cpp:740-797); the event, when given, is recorded before the final wait
+int rho_assembly(sqg_ctx *c, double lf, double *rho, double *Ebs, double *Eent, int *neg, hipEvent_t tend)
+{
+    const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
+    const int bs = 256, nb = (int)((N + bs - 1) / bs);
+    HIPCHK(c->part.ensure((size_t)2 * nb * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->st));
+    hipLaunchKernelGGL(k_rho_energy, dim3(nb), dim3(bs), 0, c->st, c->W.as<double>(), c->chi_all.as<double>(),
+                       c->zee_all.as<double>(), c->ynode.as<double>(), c->nn, c->p.npl, lf, c->p.bet,
+                       std::pow(c->p.h, 3), c->rho_d.as<double>(), c->part.as<double>(), c->flags.as<int>());
+    HIPCHK(c->Ci_all.ensure(N * K * sizeof(double)));
+    hipLaunchKernelGGL(k_ci, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->chi_all.as<double>(),
+                       c->zee_all.as<double>(), c->ynode.as<double>(), c->costab.as<double>(), c->nn, c->p.npl, lf,
+                       c->p.bet, c->Ci_all.as<double>());
+    std::vector<double> part(2 * (size_t)nb);
+    HIPCHK(hipMemcpyAsync(part.data(), c->part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(neg, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(rho, c->rho_d.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    if (tend) HIPCHK(hipEventRecord(tend, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double e1 = 0.0, e2 = 0.0;
+    for (int i = 0; i < nb; ++i) { e1 += part[2 * i]; e2 += part[2 * i + 1]; }
+    *Ebs = e1;
+    *Eent = e2;
+    return SQG_OK;
+}
+
+#include "sq_adjoint_host.hpp"
 
 double event_sum_ms(sqg_ctx *c, int which, double *bytes, long *launches)
 {
@@ -880,26 +937,10 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     }
     if (st) return st;
     HIPCHK(hipEventRecord(t2, c->st));
-    // rho, Ebs, Eent and Fermi coefficients (sq.cpp:740-797)
-    const int bs = 256, nb = (int)((N + bs - 1) / bs);
-    HIPCHK(c->part.ensure((size_t)2 * nb * sizeof(double)));
-    HIPCHK(hipMemsetAsync(c->flags.p, 0, sizeof(int), c->st));
-    hipLaunchKernelGGL(k_rho_energy, dim3(nb), dim3(bs), 0, c->st, c->W.as<double>(), c->chi_all.as<double>(),
-                       c->zee_all.as<double>(), c->ynode.as<double>(), c->nn, c->p.npl, lf, c->p.bet,
-                       std::pow(c->p.h, 3), c->rho_d.as<double>(), c->part.as<double>(), c->flags.as<int>());
-    HIPCHK(c->Ci_all.ensure(N * K * sizeof(double)));
-    hipLaunchKernelGGL(k_ci, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->chi_all.as<double>(),
-                       c->zee_all.as<double>(), c->ynode.as<double>(), c->costab.as<double>(), c->nn, c->p.npl, lf,
-                       c->p.bet, c->Ci_all.as<double>());
-    std::vector<double> part(2 * (size_t)nb);
-    int neg = 0;
-    HIPCHK(hipMemcpyAsync(part.data(), c->part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(&neg, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(rho, c->rho_d.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipEventRecord(t3, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
     double Ebs = 0.0, Eent = 0.0;
-    for (int i = 0; i < nb; ++i) { Ebs += part[2 * i]; Eent += part[2 * i + 1]; }
+    int neg = 0;
+    st = rho_assembly(c, lf, rho, &Ebs, &Eent, &neg, t3);
+    if (st) return st;
     *lambda_f = lf;
     c->lambda_f = lf;
     if (ebs) *ebs = Ebs;
@@ -1056,6 +1097,108 @@ int sqg_last_kernel_timing(sqg_ctx *c, int which, long *launches, double *total_
     if (launches) *launches = c->last_launches[which];
     if (total_ms) *total_ms = c->last_ms[which];
     if (alg_bytes) *alg_bytes = c->last_bytes[which];
+    return SQG_OK;
+}
+
+int sqg_density_fixed(sqg_ctx *c, const double *veff, double ncharge, double lf_lo, double lf_hi,
+                      double *lambda_f, double *rho)
+{
+    if (!c || !veff || !lambda_f || !rho) return SQG_E_ARG;
+    if (!c->have_state) return SQG_E_NOSTATE;
+    HIPCHK(hipSetDevice(c->p.device));
+    int st = upload_veff(c, veff);
+    if (st) return st;
+    c->have_state = false;
+    const int cap = choose_batch(c, c->nn);
+    const std::vector<int> all = node_order(c->p.n, cap);
+    long passes = 0;
+    st = moments_dispatch(c, all.data(), c->nn, &passes, cap, false);
+    if (st) return st;
+    const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
+    HIPCHK(c->W.ensure(N * K * sizeof(double)));
+    hipLaunchKernelGGL(k_cheb_W, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->mu_all.as<double>(),
+                       c->costab.as<double>(), c->nn, c->p.npl, c->W.as<double>());
+    double lf = 0.0;
+    int fit = 0;
+    st = brent(c, lf_lo, lf_hi, 1, 0.0, ncharge, &lf, &fit);
+    if (st) return st;
+    double Ebs = 0.0, Eent = 0.0;
+    int neg = 0;
+    st = rho_assembly(c, lf, rho, &Ebs, &Eent, &neg, nullptr);
+    if (st) return st;
+    *lambda_f = lf;
+    c->lambda_f = lf;
+    c->have_state = true;
+    return neg ? SQG_E_NEG_RHO : SQG_OK;
+}
+
+int sqg_adjoint_veff(sqg_ctx *c, const double *w, int fixed_n, double *dLdV)
+{
+    if (!c || !w || !dLdV) return SQG_E_ARG;
+    if (!c->have_state) return SQG_E_NOSTATE;
+    HIPCHK(hipSetDevice(c->p.device));
+    const int n = c->p.n, nn = c->nn;
+    std::vector<double> sgn, dl;
+    int st = adj_node_terms(c, sgn, dl);
+    if (st) return st;
+    // fixed electron count: the Fermi level moves by dlambda = -sum dRho/sum d rho/d lambda,
+    // which turns the weights into w - kappa
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nn; ++i) {
+        num += w[i] * sgn[i] * dl[i];
+        den += sgn[i] * dl[i];
+    }
+    if (fixed_n && den == 0.0) return SQG_E_ARG;
+    const double kappa = fixed_n ? num / den : 0.0;
+    std::vector<double> wt(nn);
+    for (int i = 0; i < nn; ++i) wt[i] = (w[i] - kappa) * sgn[i];
+    HIPCHK(c->adj_wt.ensure((size_t)nn * sizeof(double)));
+    HIPCHK(c->adj_out.ensure((size_t)nn * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->adj_wt.p, wt.data(), (size_t)nn * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->adj_out.p, 0, (size_t)nn * sizeof(double), c->st));
+    const AdjPlan P = adj_plan(c);
+    const int plane = n * n;
+    const int cap = choose_batch(c, nn, P.nvec + 2);
+    int nz = std::max(1, std::min(n, cap / plane));
+    nz = std::min(nz, std::max(1, 65535 / plane));
+    for (int z0 = 0; z0 < n; z0 += nz) {
+        const int nzb = std::min(nz, n - z0);
+        st = c->p.precision == SQG_MIXED ? adjoint_batch<float>(c, z0, nzb, P) : adjoint_batch<double>(c, z0, nzb, P);
+        if (st) return st;
+    }
+    HIPCHK(hipMemcpy(dLdV, c->adj_out.p, (size_t)nn * sizeof(double), hipMemcpyDeviceToHost));
+    return SQG_OK;
+}
+
+int sqg_tangent_veff(sqg_ctx *c, const double *dV, int fixed_n, double *drho)
+{
+    if (!c || !dV || !drho) return SQG_E_ARG;
+    if (!c->have_state) return SQG_E_NOSTATE;
+    HIPCHK(hipSetDevice(c->p.device));
+    const int nn = c->nn;
+    std::vector<double> sgn, dl;
+    int st = adj_node_terms(c, sgn, dl);
+    if (st) return st;
+    HIPCHK(c->adj_in.ensure((size_t)nn * sizeof(double)));
+    HIPCHK(c->adj_out.ensure((size_t)nn * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->adj_in.p, dV, (size_t)nn * sizeof(double), hipMemcpyHostToDevice, c->st));
+    const int cap = std::min(choose_batch(c, nn, 8), 65535);
+    for (int s0 = 0; s0 < nn; s0 += cap) {
+        const int B = std::min(cap, nn - s0);
+        st = c->p.precision == SQG_MIXED ? tangent_batch<float>(c, s0, B) : tangent_batch<double>(c, s0, B);
+        if (st) return st;
+    }
+    HIPCHK(hipMemcpy(drho, c->adj_out.p, (size_t)nn * sizeof(double), hipMemcpyDeviceToHost));
+    if (fixed_n) {
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < nn; ++i) {
+            num += drho[i];
+            den += sgn[i] * dl[i];
+        }
+        if (den == 0.0) return SQG_E_ARG;
+        const double dlam = -num / den;
+        for (int i = 0; i < nn; ++i) drho[i] += sgn[i] * dl[i] * dlam;
+    }
     return SQG_OK;
 }
 

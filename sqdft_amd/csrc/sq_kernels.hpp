@@ -101,6 +101,7 @@ struct ZArgs {
     const T *veff;                // [n^3]
     T *X;                         // input (HSUB/CHEB: t_k; LANCZOS: r')
     T *Y;                         // HSUB: out; CHEB: t_{k-1} in / t_{k+1} out; LANCZOS: v
+    T *O;                         // CHEB: t_{k+1} goes here instead of Y when set
     // CHEB per node
     const double *chi, *zee;      // indexed by batch slot, or by node id if chi_by_node
     int chi_by_node;
@@ -180,6 +181,7 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     }
     const T *X = A.X;
     T *Y = A.Y;
+    T *Yo = (MODE == MODE_CHEB && A.O) ? A.O : A.Y;
     const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
     if (SLDS)
         for (int i = tid; i < A.nl.n_ip; i += blockDim.x) s_lds[i] = sIn[i];
@@ -309,11 +311,11 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
                         o0 = (hv0 - chi * c0) * two_izee - tp0;
                         o1 = (hv1 - chi * c1) * two_izee - tp1;
                     }
-                    Y[ozp] = o0;
+                    Yo[ozp] = o0;
                     acc[0] += (double)o0 * (double)o0;
                     acc[1] += (double)o0 * (double)c0;
                     if (has1) {
-                        Y[ozp + 1] = o1;
+                        Yo[ozp + 1] = o1;
                         acc[0] += (double)o1 * (double)o1;
                         acc[1] += (double)o1 * (double)c1;
                     }

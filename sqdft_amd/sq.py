@@ -29,7 +29,7 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 
 class SpectralQuadrature:
     def __init__(self, system: System, precision: str = "fp64", batch_nodes: int = 0,
-                 device: int = 0, ckpt_interval: int = 8):
+                 device: int = 0, ckpt_interval: int = 8, fermi_tol: float | None = None):
         self._lib = L.load()
         self.system = system
         info = system.info
@@ -50,7 +50,7 @@ class SpectralQuadrature:
         p.npl = info["npl"]
         p.bet = info["bet"]
         p.lanczos_tol = info["lanczos_tol"]
-        p.fermi_tol = info["fermi_tol"]
+        p.fermi_tol = info["fermi_tol"] if fermi_tol is None else fermi_tol
         p.fermi_alg = info["fermi_alg"]
         p.precision = PRECISIONS[precision]
         p.batch_nodes = batch_nodes
@@ -141,6 +141,40 @@ class SpectralQuadrature:
         f = np.zeros(3 * na)
         L.check("sqg_forces_nonlocal", self._lib.sqg_forces_nonlocal(self._ctx, L.dptr(f)))
         return f.reshape(3, na).T.copy()
+
+    # ------------------------------------------------------------------
+    # Veff derivatives (chi, zee of the last density step frozen)
+    def density_fixed(self, veff, lf_lo: float, lf_hi: float, ncharge: float | None = None) -> dict:
+        """Density step with the last step's Lanczos bounds kept; Brent in [lf_lo, lf_hi]."""
+        v = self._veff(veff)
+        if ncharge is None:
+            ncharge = -float(self.system.total_valence())
+        lf = C.c_double()
+        rho = np.zeros(self.n ** 3)
+        L.check("sqg_density_fixed", self._lib.sqg_density_fixed(
+            self._ctx, L.dptr(v), float(ncharge), float(lf_lo), float(lf_hi), C.byref(lf), L.dptr(rho)))
+        self.lambda_f = lf.value
+        return {"rho": rho, "lambda_f": lf.value}
+
+    def adjoint_veff(self, w, fixed_n: bool = True) -> np.ndarray:
+        """dL/dVeff for L = sum_i w_i rho_i at the state of the last density step."""
+        wv = np.ascontiguousarray(np.asarray(w, dtype=np.float64).reshape(-1))
+        if wv.size != self.n ** 3:
+            raise ValueError(f"w must have n^3 = {self.n ** 3} entries")
+        out = np.zeros(self.n ** 3)
+        L.check("sqg_adjoint_veff", self._lib.sqg_adjoint_veff(self._ctx, L.dptr(wv), int(bool(fixed_n)),
+                                                               L.dptr(out)))
+        return out
+
+    def tangent_veff(self, dv, fixed_n: bool = True) -> np.ndarray:
+        """d rho along dv at the state of the last density step (forward mode)."""
+        d = np.ascontiguousarray(np.asarray(dv, dtype=np.float64).reshape(-1))
+        if d.size != self.n ** 3:
+            raise ValueError(f"dv must have n^3 = {self.n ** 3} entries")
+        out = np.zeros(self.n ** 3)
+        L.check("sqg_tangent_veff", self._lib.sqg_tangent_veff(self._ctx, L.dptr(d), int(bool(fixed_n)),
+                                                               L.dptr(out)))
+        return out
 
     def kernel_timing(self, which: int = 0) -> dict:
         """which: 0 Chebyshev recurrence z-march, 1 Lanczos z-march (last density call)."""

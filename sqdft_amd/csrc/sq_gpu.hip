@@ -66,7 +66,9 @@ struct sqg_ctx {
     std::vector<int> h_nlocc, h_nproj, h_img_type, h_img_atom, h_img_nn, h_ip_base;
     std::vector<long> h_udv_off;
     std::vector<double> h_denom;
-    DBuf img_type, img_nn, img_atom, ip_base, nlocc, nproj, udv_off, udv, inv_denom, csr_ptr, csr_ip, csr_u;
+    DBuf img_type, img_nn, img_atom, ip_base, nlocc, nproj, udv_off, udv, inv_denom;
+    DBuf ell_cnt, ell_u, ell_ip;   // projector ELL over the extended grid
+    int ell_k = 0;                 // largest entry count of any point
     DBuf Dm, contrib, ncontrib;   // nonlocal forces
     // batch buffers
     int Bcap = 0;
@@ -120,7 +122,28 @@ Geo make_geo(const sqg_ctx *c, const int *dnodes, const int *ddone)
     return g;
 }
 
-int zmarch_threads(int M) { return (((M + 1) / 2 * M) + 63) / 64 * 64; }
+// Shape of the z-march workgroup: columns per thread (CPT) and launch bound
+// (LB).  FP32: one column per thread while a plane fits 1024 threads (128
+// VGPRs per lane), else two.  FP64 (twice the registers per value): as many
+// columns as keep the workgroup within 512 threads, so that a lane may use
+// 256 VGPRs.
+struct ZShape { int cpt, lb; };
+ZShape zmarch_shape(int M, bool fp64)
+{
+    auto thr = [&](int cpt) { return (((M + cpt - 1) / cpt * M) + 63) / 64 * 64; };
+    if (fp64) {
+        for (int cpt = 2; cpt < 4; ++cpt)
+            if (thr(cpt) <= 512) return {cpt, 512};
+        return {4, 512};   // M <= 43 (MAXM) fits 512 threads at four columns
+    }
+    if (thr(1) <= 1024) return {1, 1024};
+    return {2, thr(2) <= 640 ? 640 : 1024};
+}
+int zmarch_threads(int M, bool fp64)
+{
+    const ZShape z = zmarch_shape(M, fp64);
+    return (((M + z.cpt - 1) / z.cpt * M) + 63) / 64 * 64;
+}
 
 // Visiting order of the FD nodes.  Nodes are enumerated in 4x4x2 bricks so
 // that cubes processed at the same time overlap, then each batch of B
@@ -197,9 +220,11 @@ ZArgs<T> zargs(sqg_ctx *c, const int *dn, const int *dd)
 {
     ZArgs<T> A{};
     A.g = make_geo(c, dn, dd);
-    A.nl.ptr = c->csr_ptr.as<int>();
-    A.nl.ip = c->csr_ip.as<int>();
-    A.nl.u = c->csr_u.as<T>();
+    A.nl.cnt = c->ell_cnt.as<unsigned char>();
+    A.nl.u = c->ell_u.as<T>();
+    A.nl.ip = c->ell_ip.as<int>();
+    A.nl.E3 = (size_t)c->E * c->E * c->E;
+    A.nl.k_alloc = std::max(c->ell_k, 1);
     A.nl.s = c->s.as<double>();
     A.nl.n_ip = std::max(c->n_ip, 1);
     A.veff = c->veff.as<T>();
@@ -241,26 +266,53 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
         hipLaunchKernelGGL((k_proj<T, 16, FORMED>), dim3(B), dim3(256), 0, c->st, A);
 }
 
+// LDS bytes of one z-march workgroup: two staged planes, the reduction
+// scratch and the node's projections
+size_t zmarch_lds(const sqg_ctx *c)
+{
+    return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double) +
+           (size_t)std::max(c->n_ip, 1) * sizeof(double);
+}
+
+template <typename T, int MODE, int CPT, int LB, int K>
+void zmarch_go(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
+{
+    auto fn = k_zmarch<T, MODE, CPT, K, LB>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(fn, dim3(B), dim3(zmarch_threads(c->M, sizeof(T) == 8)), shm, c->st, A);
+}
+
+// K: ELL slots read per point (the largest count, rounded up); 0 = loop
+template <typename T, int MODE, int CPT, int LB>
+void zmarch_k(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
+{
+    switch (c->ell_k <= 4 ? 4 : c->ell_k <= 8 ? 8 : c->ell_k <= 12 ? 12 : 0) {
+    case 4: zmarch_go<T, MODE, CPT, LB, 4>(c, B, A, shm); break;
+    case 8: zmarch_go<T, MODE, CPT, LB, 8>(c, B, A, shm); break;
+    case 12: zmarch_go<T, MODE, CPT, LB, 12>(c, B, A, shm); break;
+    default: zmarch_go<T, MODE, CPT, LB, 0>(c, B, A, shm); break;
+    }
+}
+
 template <typename T, int MODE>
 void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_bytes)
 {
-    const size_t nip = (size_t)std::max(c->n_ip, 1);
-    const bool slds = nip * sizeof(double) <= 48 * 1024;
-    const size_t shm = 2 * (size_t)c->plane * sizeof(T) + 16 * 4 * sizeof(double) +
-                       (slds ? nip * sizeof(double) : 0);
+    const size_t shm = zmarch_lds(c);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (which >= 0) {
         e0 = c->ev_get();
         e1 = c->ev_get();
         if (e0) (void)hipEventRecord(e0, c->st);
     }
-    const int nt = zmarch_threads(c->M);
-    if (nt <= 640) {
-        if (slds) hipLaunchKernelGGL((k_zmarch<T, MODE, 640, 1>), dim3(B), dim3(nt), shm, c->st, A);
-        else hipLaunchKernelGGL((k_zmarch<T, MODE, 640, 0>), dim3(B), dim3(nt), shm, c->st, A);
+    const ZShape zs = zmarch_shape(c->M, sizeof(T) == 8);
+    if constexpr (sizeof(T) == 8) {
+        if (zs.cpt == 2) zmarch_k<T, MODE, 2, 512>(c, B, A, shm);
+        else if (zs.cpt == 3) zmarch_k<T, MODE, 3, 512>(c, B, A, shm);
+        else zmarch_k<T, MODE, 4, 512>(c, B, A, shm);
     } else {
-        if (slds) hipLaunchKernelGGL((k_zmarch<T, MODE, 1024, 1>), dim3(B), dim3(nt), shm, c->st, A);
-        else hipLaunchKernelGGL((k_zmarch<T, MODE, 1024, 0>), dim3(B), dim3(nt), shm, c->st, A);
+        if (zs.cpt == 1) zmarch_k<T, MODE, 1, 1024>(c, B, A, shm);
+        else if (zs.lb == 640) zmarch_k<T, MODE, 2, 640>(c, B, A, shm);
+        else zmarch_k<T, MODE, 2, 1024>(c, B, A, shm);
     }
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
@@ -613,7 +665,7 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
         Z.center = c->center.as<double>(); Z.dots = c->dots.as<double>();
         Z.D = D; Z.Ci = c->Ci_all.as<double>(); Z.k = k; Z.npl = c->p.npl;
         Z.inv_h3 = 1.0 / std::pow(c->p.h, 3);
-        launch_zmarch<T, MODE_CHEB>(c, B, Z, -1, 0.0);
+        launch_zmarch<T, MODE_CHEBD>(c, B, Z, -1, 0.0);
     }
     FArgs<T> F{};
     F.g = make_geo(c, dn, nullptr);
@@ -774,10 +826,9 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
         ip += nproj[t];
     }
     c->n_ip = ip;
-    // projector CSR over the extended grid [-nloc, n-1+nloc]^3
+    // projector ELL over the extended grid [-nloc, n-1+nloc]^3
     const int E = c->E, nloc = c->p.nloc;
     const size_t E3 = (size_t)E * E * E;
-    std::vector<int> ptr(E3 + 1, 0);
     auto for_each_point = [&](auto &&fn) {
         for (int J = 0; J < n_img; ++J) {
             const int t = img_type[J], nc = nlocc[t], np = nproj[t];
@@ -801,21 +852,23 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
                     }
         }
     };
-    for_each_point([&](int, int np, size_t g, long, long) { ptr[g + 1] += np; });
-    for (size_t g = 0; g < E3; ++g) ptr[g + 1] += ptr[g];
-    const size_t ne = (size_t)ptr[E3];
-    std::vector<int> eip(std::max<size_t>(ne, 1));
-    std::vector<double> eu(std::max<size_t>(ne, 1));
-    {
-        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        for_each_point([&](int J, int np, size_t g, long q, long w3) {
-            for (int i = 0; i < np; ++i) {
-                const int k = fill[g]++;
-                eip[k] = c->h_ip_base[J] + i;
-                eu[k] = udv[c->h_udv_off[J] + i * w3 + q];
-            }
-        });
-    }
+    std::vector<int> cnt(E3, 0);
+    for_each_point([&](int, int np, size_t g, long, long) { cnt[g] += np; });
+    int kmax = 0;
+    for (size_t g = 0; g < E3; ++g) kmax = std::max(kmax, cnt[g]);
+    if (kmax > 255) return SQG_E_OVERFLOW;
+    const int KE = std::max(kmax, 1);
+    std::vector<unsigned char> ecnt(E3, 0);
+    std::vector<int> eip((size_t)KE * E3, 0);
+    std::vector<double> eu((size_t)KE * E3, 0.0);
+    for_each_point([&](int J, int np, size_t g, long q, long w3) {
+        for (int i = 0; i < np; ++i) {
+            const int k = ecnt[g]++;
+            eip[(size_t)k * E3 + g] = c->h_ip_base[J] + i;
+            eu[(size_t)k * E3 + g] = udv[c->h_udv_off[J] + i * w3 + q];
+        }
+    });
+    c->ell_k = kmax;
     const bool mixed = c->p.precision == SQG_MIXED;
     auto up = [&](DBuf &d, const void *src, size_t bytes) -> int {
         HIPCHK(d.ensure(bytes));
@@ -831,21 +884,23 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
     st |= up(c->img_atom, img_atom, std::max(n_img, 1) * sizeof(int));
     st |= up(c->ip_base, c->h_ip_base.data(), std::max(n_img, 1) * sizeof(int));
     st |= up(c->udv_off, c->h_udv_off.data(), std::max(n_img, 1) * sizeof(long));
-    st |= up(c->csr_ptr, ptr.data(), ptr.size() * sizeof(int));
-    st |= up(c->csr_ip, eip.data(), eip.size() * sizeof(int));
+    st |= up(c->ell_cnt, ecnt.data(), ecnt.size());
+    st |= up(c->ell_ip, eip.data(), eip.size() * sizeof(int));
     if (mixed) {
         std::vector<float> uf(std::max<long>(off, 1)), ef(eu.size());
         for (long i = 0; i < off; ++i) uf[i] = (float)udv[i];
         for (size_t i = 0; i < eu.size(); ++i) ef[i] = (float)eu[i];
         st |= up(c->udv, uf.data(), uf.size() * sizeof(float));
-        st |= up(c->csr_u, ef.data(), ef.size() * sizeof(float));
+        st |= up(c->ell_u, ef.data(), ef.size() * sizeof(float));
     } else {
         std::vector<double> ud(udv, udv + off);
         if (ud.empty()) ud.push_back(0.0);
         st |= up(c->udv, ud.data(), ud.size() * sizeof(double));
-        st |= up(c->csr_u, eu.data(), eu.size() * sizeof(double));
+        st |= up(c->ell_u, eu.data(), eu.size() * sizeof(double));
     }
     if (st) return SQG_E_HIP;
+    // the z-march keeps every projection of its node in LDS (160 KB per CU)
+    if (zmarch_lds(c) > 160 * 1024) return SQG_E_OVERFLOW;
     c->geom = true;
     return SQG_OK;
 }

@@ -5,7 +5,7 @@
 //   projections s       [B][n_ip] FP64, one value per image-projector
 //   image lists         [B][LCAP] image ids whose rc-cube meets the node cube
 //   Veff                [n^3] periodic grid (x fastest), read through wrapped indices
-//   projector CSR       over the extended grid (n + 2 nloc)^3: per point the
+//   projector ELL       over the extended grid (n + 2 nloc)^3: per point the
 //                       (image-projector id, U value) pairs that cover it
 // One workgroup owns one node cube: it marches through the cube along z,
 // keeping a 13-plane register window per column (z-neighbours), staging the
@@ -77,22 +77,28 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *red /* [16][N
 // ---------------------------------------------------------------- arguments
 struct Geo {
     int n, nloc, M, M2, M3, E;    // E = n + 2 nloc (extended grid side)
-    int pitch, plane;             // LDS plane pitch (even) and size
+    int pitch, plane;             // LDS plane pitch and size
     double lap[FDN + 1];          // coeff_lap
     const int *node_list;         // [B] node ids of this batch
     const int *done;              // [B] Lanczos freeze flags (nullable)
 };
 
+// Nonlocal apply data: for every point of the extended grid
+// [-nloc, n - 1 + nloc]^3 the (image-projector id, U value) pairs of the
+// projectors that cover it, in ELL form: slot j of point g at [j][g], with
+// cnt[g] valid slots (images in list order, projectors in lm order).
 template <typename T>
-struct NL {                       // nonlocal apply data
-    const int *ptr;               // [E^3 + 1]
-    const int *ip;                // entries: image-projector id
-    const T *u;                   // entries: U value
+struct NL {
+    const unsigned char *cnt;     // [E^3]
+    const T *u;                   // [K][E^3]
+    const int *ip;                // [K][E^3]
+    size_t E3;
+    int k_alloc;                  // K rows allocated
     const double *s;              // [B][n_ip] projections of the input vector
     int n_ip;
 };
 
-enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2 };
+enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3 };   // CHEBD: CHEB + D column
 
 template <typename T>
 struct ZArgs {
@@ -111,51 +117,111 @@ struct ZArgs {
     // LANCZOS per node
     const double *ca, *ib, *bprev;  // [B] alpha_{k-1}, 1/beta_k, beta_k
     double *lred;                 // [B][4] v.w, r.r, r.v, v.v
-    // CHEB with a density-matrix column (forces.cpp:410-460):
+    // CHEBD: CHEB with a density-matrix column (forces.cpp:410-460):
     // D += Ci[nd][k] * t_k / h^3 for the produced t_k
-    T *D;                         // [B][M^3] or nullptr
+    T *D;                         // [B][M^3]
     const double *Ci;             // [nn][npl+1]
     int k, npl;
     double inv_h3;
 };
 
 // ---------------------------------------------------------------- z-march
-// One workgroup per node cube.  Thread (px, y) owns the two columns
-// x0 = 2 px and x0 + 1 of row y for every z.
-// LB is the launch bound: 640 threads cover cubes up to M = 35 and leave
-// ~168 VGPRs per lane (3 waves/SIMD); 1024 covers M <= 43 with 128.
-// SLDS stages the node's projections in LDS (when they fit).
-template <typename T, int MODE, int LB, int SLDS>
+// Raw buffer loads with a per-workgroup resource: a lane offset at or past
+// the range returns 0 without touching memory, which makes every load of
+// the march branch-free (cube edges, idle lanes and empty ELL slots).
+constexpr unsigned OOB = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, size_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ int bload_i32(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ int bload_u8(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
+// One workgroup per node cube.  Thread (px, y) owns the CPT columns
+// x = CPT px + c of row y for every z.  The 13 z-neighbours of each column
+// live in a register window that shifts by one plane per step; the current
+// plane is staged in LDS for the x/y-neighbours.  Each step consumes one
+// prefetch set (loaded during the previous step: window feed z + 6,
+// pointwise inputs and nonlocal ELL entries of plane z, ELL count of plane
+// z + 1) and fills the other; the loop is unrolled by two so that the two
+// sets keep their registers and no copy waits on a load in flight.  The
+// per-plane barrier orders LDS traffic only.  K is the number of ELL slots
+// read per point (K = 0: a loop over the point's count).  The node's
+// projections sit in LDS.
+template <typename T, int MODE, int CPT, int K, int LB>
 __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
 {
-    using P2 = typename Pair<T>::type;
+    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
+    constexpr int KS = K > 0 ? K : 1;
+    constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
     const int b = blockIdx.x;
     if (G.done && G.done[b]) return;
     const int node = G.node_list[b];
     const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, E = G.E, PP = G.pitch;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
-    const int npx = (M + 1) >> 1;
+    const int npx = (M + CPT - 1) / CPT;
     const int tid = threadIdx.x;
     const int py = tid / npx;
     const int px = tid - py * npx;
     const bool act = py < M;
-    const int x0 = 2 * px;
-    const bool has1 = act && (x0 + 1 < M);
+    const int x0 = CPT * px;
+    bool has[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) has[c] = act && (x0 + c < M);
 
     // LDS: two staged planes | reduction scratch | the node's projections
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
-    double *s_lds = red + 16 * 4;
+    double *sv = red + 16 * 4;
     __shared__ double sh_center;
     for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (T)0;
     if (tid == 0) sh_center = 0.0;
+    {
+        const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
+        for (int i = tid; i < A.nl.n_ip; i += blockDim.x) sv[i] = sIn[i];
+    }
 
+    // buffer resources of this node's vectors and the shared tables
     const size_t base = (size_t)b * G.M3;
-    const int col = py * M + x0;                 // offset of the pair in a plane
-    const int gx0 = wrapn(ni - nloc + x0, n), gx1 = wrapn(ni - nloc + x0 + 1, n);
+    const size_t vb = (size_t)G.M3 * TS;
+    T *Xw = A.X + base;
+    T *Yb = A.Y + base;
+    T *Yob = ((CH && A.O) ? A.O : A.Y) + base;
+    T *Db = MODE == MODE_CHEBD ? A.D + base : nullptr;
+    const __amdgpu_buffer_rsrc_t rX = brsrc(Xw, vb), rY = brsrc(Yb, vb);
+    const __amdgpu_buffer_rsrc_t rD = brsrc(MODE == MODE_CHEBD ? Db : Yb, vb);
+    const __amdgpu_buffer_rsrc_t rV = brsrc(A.veff, (size_t)n * n * n * TS);
+    const int E3 = (int)A.nl.E3;
+    const int KA = A.nl.k_alloc;
+    const __amdgpu_buffer_rsrc_t rEU = brsrc(A.nl.u, (size_t)KA * E3 * TS);
+    const __amdgpu_buffer_rsrc_t rEI = brsrc(A.nl.ip, (size_t)KA * E3 * 4);
+    const __amdgpu_buffer_rsrc_t rEC = brsrc(A.nl.cnt, (size_t)E3);
+
+    const int col = py * M + x0;                 // offset of the first column in a plane
+    int gx[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) gx[c] = wrapn(ni - nloc + x0 + c, n);
     const int gyo = wrapn(nj - nloc + py, n) * n;
+    const int EE = E * E;
+    const int ge0 = (nk * E + (nj + py)) * E + (ni + x0);   // extended-grid point at z = 0
     const double hl = -0.5;
     const T c0d = (T)(3 * G.lap[0]);
     T cl[FDN + 1];
@@ -165,12 +231,12 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     // per-node scalars
     T chi = 0, izee = 0, two_izee = 0, ca = 0, ib = 0, bp = 0;
     T dk = 0, d0 = 0;   // DM-column weights of t_k (and t_0 on the first step)
-    if (MODE == MODE_CHEB) {
+    if (CH) {
         const int ci_idx = A.chi_by_node ? node : b;
         chi = (T)A.chi[ci_idx];
         izee = (T)(1.0 / A.zee[ci_idx]);
         two_izee = (T)(2.0 / A.zee[ci_idx]);
-        if (A.D) {
+        if (MODE == MODE_CHEBD) {
             const double *ci = A.Ci + (size_t)node * (A.npl + 1);
             dk = (T)(ci[A.k] * A.inv_h3);
             d0 = (T)(ci[0] * A.inv_h3);
@@ -179,187 +245,167 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     if (MODE == MODE_LANCZOS) {
         ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = (T)A.bprev[b];
     }
-    const T *X = A.X;
-    T *Y = A.Y;
-    T *Yo = (MODE == MODE_CHEB && A.O) ? A.O : A.Y;
-    const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
-    if (SLDS)
-        for (int i = tid; i < A.nl.n_ip; i += blockDim.x) s_lds[i] = sIn[i];
-    const double *sv = SLDS ? s_lds : sIn;
+    const bool cheb_prev = CH && !A.first;
+    const bool dacc = MODE == MODE_CHEBD && !A.first;
 
-    // input value at plane q for the stencil window (LANCZOS forms v on the fly)
-    auto load_in = [&](int q, int x, T &pv) -> T {
-        const size_t o = base + (size_t)q * M2 + py * M + x;
-        if (MODE == MODE_LANCZOS) {
-            const T r = X[o];
-            pv = Y[o];
-            return (r - ca * pv) * ib;
-        }
-        pv = 0;
-        return X[o];
+    // byte offset of (plane q, column c) in a node vector, or OOB
+    auto voff = [&](int q, int c) -> unsigned {
+        return (has[c] && q >= 0 && q < M) ? (unsigned)(q * M2 + col + c) * TS : OOB;
+    };
+    // window feed: input value of plane q (LANCZOS forms v = (r' - alpha v_{k-1}) / beta)
+    auto feed = [&](int q, int c) -> T {
+        const unsigned o = voff(q, c);
+        if (MODE == MODE_LANCZOS) return (bload<T>(rX, o) - ca * bload<T>(rY, o)) * ib;
+        return bload<T>(rX, o);
     };
 
-    T w0[WIN], w1[WIN];          // input window, columns x0 and x0+1
-    T p0[WIN], p1[WIN];          // LANCZOS: previous normalized vector v_{k-1}
+    // pointwise inputs of one plane and the ELL count of the plane after;
+    // two sets alternate between the steps of the unrolled loop
+    struct PW {
+        T ve[CPT], pv[CPT], dv[CPT];   // Veff, t_{k-1} / v_{k-1}, D
+        int cn[CPT];
+    };
+    auto ecount = [&](int q, int c) -> int {
+        return bload_u8(rEC, (has[c] && q < M) ? (unsigned)(ge0 + q * EE + c) : OOB);
+    };
+    auto fill_pw = [&](int z, PW &P) {
 #pragma unroll
-    for (int s = 0; s < WIN; ++s) { w0[s] = 0; w1[s] = 0; p0[s] = 0; p1[s] = 0; }
-    if (act) {
+        for (int c = 0; c < CPT; ++c) {
+            const unsigned o = voff(z, c);
+            const int vo = (wrapn(nk - nloc + z, n) * n) * n + gyo + gx[c];
+            P.ve[c] = bload<T>(rV, (has[c] && z < M) ? (unsigned)vo * TS : OOB);
+            P.pv[c] = (cheb_prev || MODE == MODE_LANCZOS) ? bload<T>(rY, o) : (T)0;
+            P.dv[c] = dacc ? bload<T>(rD, o) : (T)0;
+            P.cn[c] = ecount(z + 1, c);
+        }
+    };
+    // ELL entries of plane z (count cz) and the nonlocal value they give
+    T eu[CPT][KS];
+    int eip[CPT][KS];
+    auto load_ell = [&](int z, const int (&cz)[CPT]) {
 #pragma unroll
-        for (int q = 0; q < FDN; ++q)
-            if (q < M) {
-                w0[q] = load_in(q, x0, p0[q]);
-                if (has1) w1[q] = load_in(q, x0 + 1, p1[q]);
+        for (int c = 0; c < CPT; ++c) {
+            const int g = ge0 + z * EE + c;
+#pragma unroll
+            for (int j = 0; j < KS; ++j) {
+                const bool ok = K > 0 && j < cz[c];
+                eu[c][j] = bload<T>(rEU, ok ? (unsigned)(j * E3 + g) * TS : OOB);
+                eip[c][j] = bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * 4u : OOB);
             }
-    }
-    // Every global load is issued one plane ahead of its use and the
-    // per-plane barrier only orders LDS traffic, so loads stay in flight
-    // across it.
-    auto pointwise = [&](int zq, T &tp0, T &tp1, T &ve0, T &ve1, int &q0, int &q1, int &q2) {
-        tp0 = tp1 = ve0 = ve1 = 0;
-        q0 = q1 = q2 = 0;
-        if (!act || zq >= M) return;
-        if (MODE == MODE_CHEB && !A.first) {
-            const size_t o = base + (size_t)zq * M2 + col;
-            tp0 = Y[o];
-            if (has1) tp1 = Y[o + 1];
         }
-        const size_t vo = ((size_t)wrapn(nk - nloc + zq, n) * n) * n + gyo;
-        ve0 = A.veff[vo + gx0];
-        if (has1) ve1 = A.veff[vo + gx1];
-        const int ge = ((nk + zq) * E + (nj + py)) * E + (ni + x0);
-        q0 = A.nl.ptr[ge];
-        q1 = A.nl.ptr[ge + 1];
-        q2 = has1 ? A.nl.ptr[ge + 2] : q1;
     };
-    auto gather = [&](int q0, int q1, int q2, T &a0, T &a1) {
-        a0 = a1 = 0;
-        for (int e = q0; e < q1; ++e) a0 += (T)sv[A.nl.ip[e]] * A.nl.u[e];
-        for (int e = q1; e < q2; ++e) a1 += (T)sv[A.nl.ip[e]] * A.nl.u[e];
+    auto ell_value = [&](int z, int c, int cz) -> T {
+        T a = 0;
+        if (K > 0) {
+#pragma unroll
+            for (int j = 0; j < KS; ++j) a += (T)sv[eip[c][j]] * eu[c][j];
+        } else {
+            const int g = ge0 + z * EE + c;
+            if (has[c] && z < M)
+                for (int j = 0; j < cz; ++j) a += (T)sv[A.nl.ip[j * E3 + g]] * A.nl.u[j * E3 + g];
+        }
+        return a;
     };
-    T nx0 = 0, nx1 = 0, np0 = 0, np1 = 0;   // plane z+6 prefetched one step ahead
-    if (act && FDN < M) {
-        nx0 = load_in(FDN, x0, np0);
-        if (has1) nx1 = load_in(FDN, x0 + 1, np1);
+
+    // window: w[c][s] holds plane z - 6 + s during step z
+    T w[CPT][WIN], nl[CPT];
+    PW SA, SB;
+    {
+        int c0[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+            for (int s = 0; s < FDN; ++s) w[c][s] = 0;
+#pragma unroll
+            for (int s = 0; s <= FDN; ++s) w[c][FDN + s] = feed(s, c);   // planes 0..6
+            c0[c] = ecount(0, c);
+        }
+        fill_pw(0, SA);
+        load_ell(0, c0);
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) nl[c] = ell_value(0, c, c0[c]);
     }
-    T tpn0, tpn1, ven0, ven1, nln0, nln1;   // next plane's pointwise inputs
-    int qn0, qn1, qn2;
-    pointwise(0, tpn0, tpn1, ven0, ven1, qn0, qn1, qn2);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
-    gather(qn0, qn1, qn2, nln0, nln1);
 
-    for (int zb = 0; zb < M; zb += WIN) {
+    // step z: S holds plane z's pointwise inputs and the count of plane z + 1;
+    // N receives those of plane z + 1.  The ELL entries and the window feed
+    // are issued first and consumed after the stencil of the same step.
+    auto step = [&](int z, PW &S, PW &N) {
+        T qf[CPT];
+        load_ell(z + 1, S.cn);
 #pragma unroll
-        for (int u = 0; u < WIN; ++u) {
-            const int z = zb + u;
-            if (z >= M) break;
-            const int sc = u;                          // slot of plane z
-            const int sn = (u + FDN) % WIN;            // slot of plane z+6
-            w0[sn] = nx0; w1[sn] = nx1; p0[sn] = np0; p1[sn] = np1;
-            const T tp0 = tpn0, tp1 = tpn1, ve0 = ven0, ve1 = ven1, nl0 = nln0, nl1 = nln1;
-            nx0 = nx1 = np0 = np1 = 0;
-            if (act && z + FDN + 1 < M) {
-                nx0 = load_in(z + FDN + 1, x0, np0);
-                if (has1) nx1 = load_in(z + FDN + 1, x0 + 1, np1);
-            }
-            pointwise(z + 1, tpn0, tpn1, ven0, ven1, qn0, qn1, qn2);
-            const size_t ozp = base + (size_t)z * M2 + col;
-            // stage the current plane
-            T *pl = lds + (z & 1) * G.plane;
-            const T c0 = w0[sc], c1 = w1[sc];
-            if (act) {
-                pl[(py + FDN) * PP + x0 + FDN] = c0;
-                if (has1) pl[(py + FDN) * PP + x0 + FDN + 1] = c1;
-            }
-            lds_barrier();
-            if (act) {
-                const T *row = pl + (py + FDN) * PP + x0;   // padded x0 .. x0+13
-                const P2 r0 = *reinterpret_cast<const P2 *>(row + 0);
-                const P2 r1 = *reinterpret_cast<const P2 *>(row + 2);
-                const P2 r2 = *reinterpret_cast<const P2 *>(row + 4);
-                const P2 r4 = *reinterpret_cast<const P2 *>(row + 8);
-                const P2 r5 = *reinterpret_cast<const P2 *>(row + 10);
-                const P2 r6 = *reinterpret_cast<const P2 *>(row + 12);
-                // padded row x0..x0+13: r0 r1 r2 | c0 c1 | r4 r5 r6
-                T t0 = c0 * c0d, t1 = c1 * c0d;
-#define SQK_TERM(A, XL0, XR0, XL1, XR1)                                                         \
-    {                                                                                           \
-        const P2 ym = *reinterpret_cast<const P2 *>(pl + (py + FDN - (A)) * PP + x0 + FDN);     \
-        const P2 yp = *reinterpret_cast<const P2 *>(pl + (py + FDN + (A)) * PP + x0 + FDN);     \
-        const int zm = (u - (A) + WIN) % WIN, zp = (u + (A)) % WIN;                             \
-        t0 += ((XL0) + (XR0) + ym.x + yp.x + w0[zm] + w0[zp]) * cl[A];                          \
-        t1 += ((XL1) + (XR1) + ym.y + yp.y + w1[zm] + w1[zp]) * cl[A];                          \
-    }
-                SQK_TERM(1, r2.y, c1, c0, r4.x)
-                SQK_TERM(2, r2.x, r4.x, r2.y, r4.y)
-                SQK_TERM(3, r1.y, r4.y, r2.x, r5.x)
-                SQK_TERM(4, r1.x, r5.x, r1.y, r5.y)
-                SQK_TERM(5, r0.y, r5.y, r1.x, r6.x)
-                SQK_TERM(6, r0.x, r6.x, r0.y, r6.y)
-#undef SQK_TERM
-                const T hv0 = (T)hl * t0 + ve0 * c0 + nl0;
-                const T hv1 = (T)hl * t1 + ve1 * c1 + nl1;
+        for (int c = 0; c < CPT; ++c) qf[c] = feed(z + FDN + 1, c);
+        fill_pw(z + 1, N);
+        // stage plane z
+        T *pl = lds + (z & 1) * G.plane;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c)
+            if (has[c]) pl[(py + FDN) * PP + x0 + c + FDN] = w[c][FDN];
+        lds_barrier();
+        if (act) {
+            const T *row = pl + (py + FDN) * PP + x0 + FDN;
+#pragma unroll
+            for (int c = 0; c < CPT; ++c) {
+                if (!has[c]) continue;
+                const T cv = w[c][FDN];
+                T t = cv * c0d;
+#pragma unroll
+                for (int a = 1; a <= FDN; ++a) {
+                    const T xl = row[c - a], xr = row[c + a];
+                    const T ym = row[c - a * PP], yp = row[c + a * PP];
+                    t += (xl + xr + ym + yp + w[c][FDN - a] + w[c][FDN + a]) * cl[a];
+                    // keep the LDS reads of one offset in flight at a time (register pressure)
+                    if (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+                }
+                const T hv = (T)hl * t + S.ve[c] * cv + nl[c];
+                const int ozp = z * M2 + col + c;
                 if (MODE == MODE_HSUB) {
-                    Y[ozp] = hv0;
-                    if (has1) Y[ozp + 1] = hv1;
-                } else if (MODE == MODE_CHEB) {
-                    T o0, o1;
-                    if (A.first) {
-                        o0 = (hv0 - chi * c0) * izee;
-                        o1 = (hv1 - chi * c1) * izee;
-                    } else {
-                        o0 = (hv0 - chi * c0) * two_izee - tp0;
-                        o1 = (hv1 - chi * c1) * two_izee - tp1;
-                    }
-                    Yo[ozp] = o0;
-                    acc[0] += (double)o0 * (double)o0;
-                    acc[1] += (double)o0 * (double)c0;
-                    if (has1) {
-                        Yo[ozp + 1] = o1;
-                        acc[0] += (double)o1 * (double)o1;
-                        acc[1] += (double)o1 * (double)c1;
-                    }
-                    if (A.D) {
-                        T *Dp = A.D + ozp;
+                    Yb[ozp] = hv;
+                } else if (CH) {
+                    T o;
+                    if (A.first) o = (hv - chi * cv) * izee;
+                    else o = (hv - chi * cv) * two_izee - S.pv[c];
+                    Yob[ozp] = o;
+                    acc[0] += (double)o * (double)o;
+                    acc[1] += (double)o * (double)cv;
+                    if (MODE == MODE_CHEBD) {
                         if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
-                            const bool cz = z == nloc && py == nloc;
-                            Dp[0] = o0 * dk + ((cz && x0 == nloc) ? d0 : (T)0);
-                            if (has1) Dp[1] = o1 * dk + ((cz && x0 + 1 == nloc) ? d0 : (T)0);
+                            const bool ctr = z == nloc && py == nloc && x0 + c == nloc;
+                            Db[ozp] = o * dk + (ctr ? d0 : (T)0);
                         } else {
-                            Dp[0] += o0 * dk;
-                            if (has1) Dp[1] += o1 * dk;
+                            Db[ozp] = S.dv[c] + o * dk;
                         }
                     }
-                    if (z == nloc && py == nloc) {
-                        if (x0 == nloc) sh_center = (double)o0;
-                        else if (x0 + 1 == nloc) sh_center = (double)o1;
-                    }
+                    if (z == nloc && py == nloc && x0 + c == nloc) sh_center = (double)o;
                 } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
-                    const T q0 = hv0 - bp * p0[sc];
-                    const T q1 = hv1 - bp * p1[sc];
-                    A.X[ozp] = q0;
-                    Y[ozp] = c0;
-                    acc[0] += (double)c0 * (double)hv0;
-                    acc[1] += (double)q0 * (double)q0;
-                    acc[2] += (double)q0 * (double)c0;
-                    acc[3] += (double)c0 * (double)c0;
-                    if (has1) {
-                        A.X[ozp + 1] = q1;
-                        Y[ozp + 1] = c1;
-                        acc[0] += (double)c1 * (double)hv1;
-                        acc[1] += (double)q1 * (double)q1;
-                        acc[2] += (double)q1 * (double)c1;
-                        acc[3] += (double)c1 * (double)c1;
-                    }
+                    const T q = hv - bp * S.pv[c];
+                    Xw[ozp] = q;
+                    Yb[ozp] = cv;
+                    acc[0] += (double)cv * (double)hv;
+                    acc[1] += (double)q * (double)q;
+                    acc[2] += (double)q * (double)cv;
+                    acc[3] += (double)cv * (double)cv;
                 }
             }
-            gather(qn0, qn1, qn2, nln0, nln1);   // projector apply of plane z+1
         }
+        // nonlocal value of plane z + 1, then shift the window and insert plane z + 7
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            nl[c] = ell_value(z + 1, c, S.cn[c]);
+#pragma unroll
+            for (int s = 0; s < WIN - 1; ++s) w[c][s] = w[c][s + 1];
+            w[c][WIN - 1] = qf[c];
+        }
+    };
+    for (int z = 0; z < M; z += 2) {
+        step(z, SA, SB);
+        if (z + 1 < M) step(z + 1, SB, SA);
     }
     if (MODE == MODE_HSUB) return;
     block_sum<4>(acc, red);
     if (tid == 0) {
-        if (MODE == MODE_CHEB) {
+        if (CH) {
             A.center[b] = sh_center;
             A.dots[2 * b] = acc[0];
             A.dots[2 * b + 1] = acc[1];

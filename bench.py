@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="before the steps, run one plain device copy of this size (MiB) so that "
+                         "counter runs can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
@@ -94,6 +97,16 @@ def main():
     sq = SpectralQuadrature(S, precision=args.precision, batch_nodes=args.batch_nodes)
     info = S.info
     nn = S.nn
+    if args.copy_calib_mib > 0:
+        import torch
+        nb = args.copy_calib_mib * (1 << 20)
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {nb} bytes read + {nb} bytes written")
+        del src, dst
     for i in range(args.warmup):
         log(f"bench: warmup {i}")
         sq.density(veff, scf_iter=1)
@@ -143,10 +156,24 @@ def main():
         mo = sqm.density(veff, scf_iter=1)
         tm = time.perf_counter() - t0
         err = float(np.max(np.abs(mo["rho"] - last["rho"])) / np.max(np.abs(last["rho"])))
-        out["mixed"] = {"value": nn / tm, "unit": "nodes/s", "ms_per_step": 1e3 * tm,
-                        "speedup_vs_fp64": (ms * 1e-3) / tm, "rho_rel_err_vs_fp64": err,
-                        "lambda_f_err": abs(mo["lambda_f"] - last["lambda_f"])}
         sqm.close()
+        # accuracy of the mixed arithmetic itself: both Fermi levels solved to
+        # 1e-12 (at fermi_tol = 1e-6 the two roots may differ by ~1e-6, which
+        # alone moves rho by a few 1e-6)
+        log("bench: mixed vs fp64 accuracy at fermi_tol 1e-12")
+        r64 = SpectralQuadrature(S, precision="fp64", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
+        t64 = r64.density(veff, scf_iter=1)
+        r64.close()
+        r32 = SpectralQuadrature(S, precision="mixed", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
+        t32 = r32.density(veff, scf_iter=1)
+        r32.close()
+        err_t = float(np.max(np.abs(t32["rho"] - t64["rho"])) / np.max(np.abs(t64["rho"])))
+        out["mixed"] = {"value": nn / tm, "unit": "nodes/s", "ms_per_step": 1e3 * tm,
+                        "speedup_vs_fp64": (ms * 1e-3) / tm,
+                        "rho_rel_err_vs_fp64": err_t,
+                        "lambda_f_err": abs(t32["lambda_f"] - t64["lambda_f"]),
+                        "rho_rel_err_at_fermi_tol_1e-6": err,
+                        "ebs_rel_err": abs(t32["ebs"] - t64["ebs"]) / abs(t64["ebs"])}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(S, veff, cfg["Rcut"], cfg["T"], args.cpu_seconds)

@@ -141,8 +141,9 @@ int adjoint_batch(sqg_ctx *c, int z0, int nz, const AdjPlan &P)
                        (const T *)tb(1), G);
     hipLaunchKernelGGL((k_adj_gather<T>), dim3(grid1(c->nn, 256)), dim3(256), 0, c->st, n, c->p.nloc, z0, nz,
                        (const T *)G, c->zee_all.as<double>(), c->adj_out.as<double>());
-    int ovf = 0;
-    HIPCHK(hipMemcpyAsync(&ovf, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    int ovf4[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(ovf4, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    const int ovf = ovf4[0] | ovf4[2] | ovf4[3];
     HIPCHK(hipStreamSynchronize(c->st));
     return ovf ? SQG_E_OVERFLOW : SQG_OK;
 }
@@ -185,8 +186,9 @@ int tangent_batch(sqg_ctx *c, int s0, int B)
     hipLaunchKernelGGL(k_tan_finish, dim3((B + 127) / 128), dim3(128), 0, c->st, B, K2, q4, c->p.npl,
                        1.0 / std::pow(c->p.h, 3), dn, (const double *)rec, c->Ci_all.as<double>(),
                        c->adj_sgn.as<double>(), c->adj_out.as<double>());
-    int ovf = 0;
-    HIPCHK(hipMemcpyAsync(&ovf, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    int ovf4[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(ovf4, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    const int ovf = ovf4[0] | ovf4[2] | ovf4[3];
     HIPCHK(hipStreamSynchronize(c->st));
     return ovf ? SQG_E_OVERFLOW : SQG_OK;
 }

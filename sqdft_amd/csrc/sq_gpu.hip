@@ -72,7 +72,7 @@ struct sqg_ctx {
     DBuf Dm, contrib, ncontrib;   // nonlocal forces
     // batch buffers
     int Bcap = 0;
-    DBuf X, Y, s, node_list, list, list_cnt, flags;
+    DBuf X, Y, NLv, s, node_list, list, list_cnt, flags;
     DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, cnt, done, ca, ib, bprev;
     DBuf chiB, zeeB, center, dots, htemp;
     // all-node state of the last density step
@@ -123,37 +123,29 @@ Geo make_geo(const sqg_ctx *c, const int *dnodes, const int *ddone)
 }
 
 // Shape of the z-march workgroup: columns per thread (CPT) and launch bound
-// (LB).  FP32: one column per thread while a plane fits 1024 threads (128
-// VGPRs per lane), else two.  FP64 (twice the registers per value): as many
-// columns as keep the workgroup within 512 threads, so that a lane may use
-// 256 VGPRs.
+// (LB).  A plane of M <= 32 fits 1024 threads at one column per thread;
+// larger cubes take two columns (M <= 35 within 640 threads) or more.
 struct ZShape { int cpt, lb; };
+int zmarch_threads_cpt(int M, int cpt) { return (((M + cpt - 1) / cpt * M) + 63) / 64 * 64; }
 ZShape zmarch_shape(int M, bool fp64)
 {
-    auto thr = [&](int cpt) { return (((M + cpt - 1) / cpt * M) + 63) / 64 * 64; };
-    if (fp64) {
-        for (int cpt = 2; cpt < 4; ++cpt)
-            if (thr(cpt) <= 512) return {cpt, 512};
-        return {4, 512};   // M <= 43 (MAXM) fits 512 threads at four columns
-    }
-    if (thr(1) <= 1024) return {1, 1024};
-    return {2, thr(2) <= 640 ? 640 : 1024};
+    (void)fp64;
+    if (zmarch_threads_cpt(M, 1) <= 1024) return {1, 1024};
+    if (zmarch_threads_cpt(M, 2) <= 640) return {2, 640};
+    if (zmarch_threads_cpt(M, 2) <= 1024 && !fp64) return {2, 1024};
+    for (int cpt = 3; cpt < 4; ++cpt)
+        if (zmarch_threads_cpt(M, cpt) <= 512) return {cpt, 512};
+    return {4, 512};   // M <= 43 (MAXM) fits 512 threads at four columns
 }
-int zmarch_threads(int M, bool fp64)
-{
-    const ZShape z = zmarch_shape(M, fp64);
-    return (((M + z.cpt - 1) / z.cpt * M) + 63) / 64 * 64;
-}
+int zmarch_threads(int M, bool fp64) { return zmarch_threads_cpt(M, zmarch_shape(M, fp64).cpt); }
 
-// Visiting order of the FD nodes.  Nodes are enumerated in 4x4x2 bricks so
-// that cubes processed at the same time overlap, then each batch of B
-// consecutive nodes is dealt out so that workgroup i (dispatched to XCD
-// i mod 8) takes the i/8-th node of the (i mod 8)-th contiguous eighth of the
-// batch: every XCD walks a compact region and its L2 serves the Veff and
-// projector data that neighbouring cubes share.
-std::vector<int> node_order(int n, int B)
+// Visiting order of the FD nodes: 4x4x2 bricks, so that the 16 consecutive
+// batch slots that one projection workgroup takes (PGS) are a compact 4x4 slab
+// whose cubes share nearly all their images, and neighbouring workgroups of
+// the z-march and of the nonlocal apply touch overlapping cubes.
+std::vector<int> node_order(int n)
 {
-    const int bx = 4, by = 4, bz = 2, NX = 8;
+    const int bx = 4, by = 4, bz = 2;
     std::vector<int> seq;
     seq.reserve((size_t)n * n * n);
     for (int z0 = 0; z0 < n; z0 += bz)
@@ -162,18 +154,7 @@ std::vector<int> node_order(int n, int B)
                 for (int z = z0; z < std::min(z0 + bz, n); ++z)
                     for (int y = y0; y < std::min(y0 + by, n); ++y)
                         for (int x = x0; x < std::min(x0 + bx, n); ++x) seq.push_back((z * n + y) * n + x);
-    std::vector<int> out(seq.size());
-    for (size_t s = 0; s < seq.size(); s += B) {
-        const size_t len = std::min(seq.size() - s, (size_t)B);
-        const size_t ch = (len + NX - 1) / NX;
-        size_t o = s;
-        for (size_t i = 0; i < ch; ++i)
-            for (int k = 0; k < NX; ++k) {
-                const size_t src = k * ch + i;
-                if (src < len) out[o++] = seq[s + src];
-            }
-    }
-    return out;
+    return seq;
 }
 
 // Allocates per-batch buffers for up to B nodes.
@@ -184,6 +165,7 @@ int ensure_batch(sqg_ctx *c, int B)
     const size_t S = MAXIT + 2;
     HIPCHK(c->X.ensure((size_t)B * c->M3 * rs));
     HIPCHK(c->Y.ensure((size_t)B * c->M3 * rs));
+    HIPCHK(c->NLv.ensure((size_t)B * c->M3 * rs));
     HIPCHK(c->s.ensure((size_t)B * std::max(c->n_ip, 1) * sizeof(double)));
     HIPCHK(c->node_list.ensure((size_t)B * sizeof(int)));
     HIPCHK(c->list.ensure((size_t)B * LCAP * sizeof(int)));
@@ -207,7 +189,7 @@ int choose_batch(sqg_ctx *c, int want, int nvec = 2)
     B = std::min(B, want);
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
-        const size_t per = nvec * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
+        const size_t per = (nvec + 1) * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
                            LCAP * 4 + 4 * (MAXIT + 2) * 8 + 256;
         const size_t cap = (size_t)(0.6 * (double)(freeb + (size_t)c->Bcap * per)) / per;
         if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
@@ -220,13 +202,7 @@ ZArgs<T> zargs(sqg_ctx *c, const int *dn, const int *dd)
 {
     ZArgs<T> A{};
     A.g = make_geo(c, dn, dd);
-    A.nl.cnt = c->ell_cnt.as<unsigned char>();
-    A.nl.u = c->ell_u.as<T>();
-    A.nl.ip = c->ell_ip.as<int>();
-    A.nl.E3 = (size_t)c->E * c->E * c->E;
-    A.nl.k_alloc = std::max(c->ell_k, 1);
-    A.nl.s = c->s.as<double>();
-    A.nl.n_ip = std::max(c->n_ip, 1);
+    A.nlv = c->NLv.as<T>();
     A.veff = c->veff.as<T>();
     return A;
 }
@@ -251,47 +227,61 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     A.inv_denom = c->inv_denom.as<double>();
     A.ca = c->ca.as<double>();
     A.ib = c->ib.as<double>();
+    A.overflow = c->flags.as<int>() + 3;
     return A;
 }
 
+template <typename T, int K>
+void nlapply_go(sqg_ctx *c, int B, const NArgs<T> &A)
+{
+    const size_t shm = (size_t)std::max(c->n_ip, 1) * sizeof(double);
+    auto fn = k_nlapply<T, K>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(fn, dim3(B), dim3(512), shm, c->st, A);
+}
+
+// Projections s = <U, v>/Denom of the B vectors (k_proj), then their
+// nonlocal term NL = sum U s into NLv (k_nlapply) for the next z-march.
 template <typename T, int FORMED>
 void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
 {
-    if (c->n_ip == 0) return;
-    if (c->max_nproj <= 1)
-        hipLaunchKernelGGL((k_proj<T, 1, FORMED>), dim3(B), dim3(256), 0, c->st, A);
-    else if (c->max_nproj <= 4)
-        hipLaunchKernelGGL((k_proj<T, 4, FORMED>), dim3(B), dim3(256), 0, c->st, A);
-    else
-        hipLaunchKernelGGL((k_proj<T, 16, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+    if (c->n_ip > 0) {
+        const size_t shm = (size_t)((c->n_img + 31) / 32) * 4 + (size_t)GLCAP * 4;
+        auto fn = k_proj_mfma<T, FORMED>;
+        if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipLaunchKernelGGL(fn, dim3((B + PGS - 1) / PGS), dim3(256), shm, c->st, A, B, c->n_img);
+    }
+    NArgs<T> N{};
+    N.g = A.g;
+    N.nl.cnt = c->ell_cnt.as<unsigned char>();
+    N.nl.u = c->ell_u.as<T>();
+    N.nl.ip = c->ell_ip.as<int>();
+    N.nl.E3 = (size_t)c->E * c->E * c->E;
+    N.nl.k_alloc = std::max(c->ell_k, 1);
+    N.nl.s = c->s.as<double>();
+    N.nl.n_ip = std::max(c->n_ip, 1);
+    N.nlv = c->NLv.as<T>();
+    switch (c->ell_k <= 4 ? 4 : c->ell_k <= 8 ? 8 : c->ell_k <= 12 ? 12 : c->ell_k <= 16 ? 16 : 0) {
+    case 4: nlapply_go<T, 4>(c, B, N); break;
+    case 8: nlapply_go<T, 8>(c, B, N); break;
+    case 12: nlapply_go<T, 12>(c, B, N); break;
+    case 16: nlapply_go<T, 16>(c, B, N); break;
+    default: nlapply_go<T, 0>(c, B, N); break;
+    }
 }
 
-// LDS bytes of one z-march workgroup: two staged planes, the reduction
-// scratch and the node's projections
+// LDS bytes of one z-march workgroup: two staged planes and the reduction scratch
 size_t zmarch_lds(const sqg_ctx *c)
 {
-    return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double) +
-           (size_t)std::max(c->n_ip, 1) * sizeof(double);
+    return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double);
 }
 
-template <typename T, int MODE, int CPT, int LB, int K>
-void zmarch_go(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
-{
-    auto fn = k_zmarch<T, MODE, CPT, K, LB>;
-    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(fn, dim3(B), dim3(zmarch_threads(c->M, sizeof(T) == 8)), shm, c->st, A);
-}
-
-// K: ELL slots read per point (the largest count, rounded up); 0 = loop
 template <typename T, int MODE, int CPT, int LB>
 void zmarch_k(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
 {
-    switch (c->ell_k <= 4 ? 4 : c->ell_k <= 8 ? 8 : c->ell_k <= 12 ? 12 : 0) {
-    case 4: zmarch_go<T, MODE, CPT, LB, 4>(c, B, A, shm); break;
-    case 8: zmarch_go<T, MODE, CPT, LB, 8>(c, B, A, shm); break;
-    case 12: zmarch_go<T, MODE, CPT, LB, 12>(c, B, A, shm); break;
-    default: zmarch_go<T, MODE, CPT, LB, 0>(c, B, A, shm); break;
-    }
+    auto fn = k_zmarch<T, MODE, CPT, LB>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(fn, dim3(B), dim3(zmarch_threads(c->M, sizeof(T) == 8)), shm, c->st, A);
 }
 
 template <typename T, int MODE>
@@ -305,15 +295,12 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         if (e0) (void)hipEventRecord(e0, c->st);
     }
     const ZShape zs = zmarch_shape(c->M, sizeof(T) == 8);
-    if constexpr (sizeof(T) == 8) {
-        if (zs.cpt == 2) zmarch_k<T, MODE, 2, 512>(c, B, A, shm);
-        else if (zs.cpt == 3) zmarch_k<T, MODE, 3, 512>(c, B, A, shm);
-        else zmarch_k<T, MODE, 4, 512>(c, B, A, shm);
-    } else {
-        if (zs.cpt == 1) zmarch_k<T, MODE, 1, 1024>(c, B, A, shm);
-        else if (zs.lb == 640) zmarch_k<T, MODE, 2, 640>(c, B, A, shm);
-        else zmarch_k<T, MODE, 2, 1024>(c, B, A, shm);
-    }
+    if (zs.cpt == 1) zmarch_k<T, MODE, 1, 1024>(c, B, A, shm);
+    else if (zs.cpt == 2 && zs.lb == 512) zmarch_k<T, MODE, 2, 512>(c, B, A, shm);
+    else if (zs.cpt == 2 && zs.lb == 640) zmarch_k<T, MODE, 2, 640>(c, B, A, shm);
+    else if (zs.cpt == 2) zmarch_k<T, MODE, 2, 1024>(c, B, A, shm);
+    else if (zs.cpt == 3) zmarch_k<T, MODE, 3, 512>(c, B, A, shm);
+    else zmarch_k<T, MODE, 4, 512>(c, B, A, shm);
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
         c->ev[which].push_back({e0, e1, alg_bytes});
@@ -444,10 +431,10 @@ int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
         Mg.k = k;
         hipLaunchKernelGGL(k_moments, dim3((B + 127) / 128), dim3(128), 0, c->st, Mg);
     }
-    int hflags[2];
-    HIPCHK(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    int hflags[4];
+    HIPCHK(hipMemcpyAsync(hflags, flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (hflags[0]) return SQG_E_OVERFLOW;
+    if (hflags[0] || hflags[3]) return SQG_E_OVERFLOW;
     if (hflags[1]) return SQG_E_QL;
     return SQG_OK;
 }
@@ -681,12 +668,12 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
     else hipLaunchKernelGGL((k_force_contract<T, 16>), dim3(B), dim3(256), 0, c->st, F);
     std::vector<double> hc((size_t)B * FSLOT * 4);
     std::vector<int> hn(B);
-    int hf[3];
+    int hf[4];
     HIPCHK(hipMemcpyAsync(hc.data(), c->contrib.p, hc.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(hn.data(), c->ncontrib.p, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipMemcpyAsync(hf, flags, 3 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(hf, flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    if (hf[0] || hf[2]) return SQG_E_OVERFLOW;
+    if (hf[0] || hf[2] || hf[3]) return SQG_E_OVERFLOW;
     for (int b = 0; b < B; ++b)
         for (int s = 0; s < hn[b]; ++s) {
             const double *o = &hc[((size_t)b * FSLOT + s) * 4];
@@ -899,8 +886,9 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
         st |= up(c->ell_u, eu.data(), eu.size() * sizeof(double));
     }
     if (st) return SQG_E_HIP;
-    // the z-march keeps every projection of its node in LDS (160 KB per CU)
-    if (zmarch_lds(c) > 160 * 1024) return SQG_E_OVERFLOW;
+    // the nonlocal apply keeps every projection of its node in LDS (160 KB per CU)
+    if ((size_t)std::max(c->n_ip, 1) * sizeof(double) > 160 * 1024) return SQG_E_OVERFLOW;
+    if ((size_t)std::max(kmax, 1) * E3 * sizeof(double) >= 0x80000000u) return SQG_E_OVERFLOW;
     c->geom = true;
     return SQG_OK;
 }
@@ -958,7 +946,7 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     c->pool_used = 0;
     HIPCHK(hipEventRecord(t0, c->st));
     const int cap = choose_batch(c, c->nn);
-    const std::vector<int> all = node_order(c->p.n, cap);
+    const std::vector<int> all = node_order(c->p.n);
     long passes = 0;
     st = moments_dispatch(c, all.data(), c->nn, &passes, cap);
     if (st) return st;
@@ -1079,8 +1067,9 @@ int sqg_hsub_apply_batch(sqg_ctx *c, const double *veff, const int *nodes, int n
     };
     st = c->p.precision == SQG_MIXED ? run(float()) : run(double());
     if (st) return st;
-    int ovf = 0;
-    HIPCHK(hipMemcpyAsync(&ovf, c->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    int ovf4[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(ovf4, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    const int ovf = ovf4[0] | ovf4[2] | ovf4[3];
     HIPCHK(hipMemcpyAsync(hv, tmp.p, len * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return ovf ? SQG_E_OVERFLOW : SQG_OK;
@@ -1096,7 +1085,7 @@ int sqg_forces_nonlocal(sqg_ctx *c, double *f)
     if (c->n_ip == 0) return SQG_OK;
     std::vector<double> rec;   // node, atom, fx, fy, fz per contributing image
     const int cap = choose_batch(c, c->nn, 3);
-    const std::vector<int> nodes = node_order(c->p.n, cap);
+    const std::vector<int> nodes = node_order(c->p.n);
     int done_n = 0;
     while (done_n < c->nn) {
         const int B = std::min(cap, c->nn - done_n);
@@ -1165,7 +1154,7 @@ int sqg_density_fixed(sqg_ctx *c, const double *veff, double ncharge, double lf_
     if (st) return st;
     c->have_state = false;
     const int cap = choose_batch(c, c->nn);
-    const std::vector<int> all = node_order(c->p.n, cap);
+    const std::vector<int> all = node_order(c->p.n);
     long passes = 0;
     st = moments_dispatch(c, all.data(), c->nn, &passes, cap, false);
     if (st) return st;

@@ -103,7 +103,7 @@ enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3 };   // CH
 template <typename T>
 struct ZArgs {
     Geo g;
-    NL<T> nl;
+    const T *nlv;                 // [B][M^3] nonlocal term of the input vector (k_nlapply)
     const T *veff;                // [n^3]
     T *X;                         // input (HSUB/CHEB: t_k; LANCZOS: r')
     T *Y;                         // HSUB: out; CHEB: t_{k-1} in / t_{k+1} out; LANCZOS: v
@@ -153,28 +153,85 @@ __device__ __forceinline__ int bload_u8(__amdgpu_buffer_rsrc_t r, unsigned off)
     return (int)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
 }
 
-// One workgroup per node cube.  Thread (px, y) owns the CPT columns
-// x = CPT px + c of row y for every z.  The 13 z-neighbours of each column
-// live in a register window that shifts by one plane per step; the current
-// plane is staged in LDS for the x/y-neighbours.  Each step consumes one
-// prefetch set (loaded during the previous step: window feed z + 6,
-// pointwise inputs and nonlocal ELL entries of plane z, ELL count of plane
-// z + 1) and fills the other; the loop is unrolled by two so that the two
-// sets keep their registers and no copy waits on a load in flight.  The
-// per-plane barrier orders LDS traffic only.  K is the number of ELL slots
-// read per point (K = 0: a loop over the point's count).  The node's
-// projections sit in LDS.
-template <typename T, int MODE, int CPT, int K, int LB>
-__global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
+// Nonlocal term of B node vectors, NL_b(p) = sum_{J,lm} U_J,lm(p) s_b[J,lm]
+// with the projections s_b of the vector (VnlocsubTimesVec, nonlocal.cpp:173-191):
+// one workgroup per node, the node's projections in LDS, one thread per
+// point at a time through the ELL entries of the extended-grid point.
+template <typename T>
+struct NArgs {
+    Geo g;
+    NL<T> nl;
+    T *nlv;                       // [B][M^3] out
+};
+
+template <typename T, int K>
+__global__ void __launch_bounds__(512) k_nlapply(NArgs<T> A)
 {
-    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
     constexpr int KS = K > 0 ? K : 1;
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
     const int b = blockIdx.x;
     if (G.done && G.done[b]) return;
     const int node = G.node_list[b];
-    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, E = G.E, PP = G.pitch;
+    const int n = G.n, M = G.M, M2 = G.M2, E = G.E;
+    const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
+    extern __shared__ unsigned char smem_raw[];
+    double *sv = reinterpret_cast<double *>(smem_raw);
+    {
+        const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
+        for (int i = threadIdx.x; i < A.nl.n_ip; i += blockDim.x) sv[i] = sIn[i];
+    }
+    __syncthreads();
+    const int E3 = (int)A.nl.E3;
+    const int KA = A.nl.k_alloc;
+    const __amdgpu_buffer_rsrc_t rEU = brsrc(A.nl.u, (size_t)KA * E3 * TS);
+    const __amdgpu_buffer_rsrc_t rEI = brsrc(A.nl.ip, (size_t)KA * E3 * 4);
+    const __amdgpu_buffer_rsrc_t rEC = brsrc(A.nl.cnt, (size_t)E3);
+    T *out = A.nlv + (size_t)b * G.M3;
+    const int g00 = (nk * E + nj) * E + ni;
+#pragma unroll 2
+    for (int p = threadIdx.x; p < G.M3; p += blockDim.x) {
+        const int z = p / M2, r = p - z * M2, y = r / M, x = r - y * M;
+        const int g = g00 + (z * E + y) * E + x;
+        const int cn = bload_u8(rEC, (unsigned)g);
+        double a = 0.0;
+        if (K > 0) {
+            T u[KS];
+            int ip[KS];
+#pragma unroll
+            for (int j = 0; j < KS; ++j) {
+                const bool ok = j < cn;
+                u[j] = bload<T>(rEU, ok ? (unsigned)(j * E3 + g) * TS : OOB);
+                ip[j] = bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * 4u : OOB);
+            }
+#pragma unroll
+            for (int j = 0; j < KS; ++j) a += sv[ip[j]] * (double)u[j];
+        } else {
+            for (int j = 0; j < cn; ++j) a += sv[A.nl.ip[j * E3 + g]] * (double)A.nl.u[j * E3 + g];
+        }
+        out[p] = (T)a;
+    }
+}
+
+// One workgroup per node cube.  Thread (px, y) owns the CPT columns
+// x = CPT px + c of row y for every z.  The 13 z-neighbours of each column
+// live in a register window that shifts by one plane per step; the current
+// plane is staged in LDS for the x/y-neighbours.  The pointwise inputs of
+// plane z (Veff, the nonlocal term, t_{k-1} or v_{k-1}, D) are loaded during
+// step z - 1 into one of two register sets that alternate between the steps
+// of the loop (unrolled by two, so no copy waits on a load in flight); the
+// window feed of plane z + 7 is issued at the start of step z and inserted
+// at its end.  The per-plane barrier orders LDS traffic only.
+template <typename T, int MODE, int CPT, int LB>
+__global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
+{
+    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
+    constexpr unsigned TS = sizeof(T);
+    const Geo &G = A.g;
+    const int b = blockIdx.x;
+    if (G.done && G.done[b]) return;
+    const int node = G.node_list[b];
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, PP = G.pitch;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
     const int npx = (M + CPT - 1) / CPT;
     const int tid = threadIdx.x;
@@ -186,20 +243,15 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
 #pragma unroll
     for (int c = 0; c < CPT; ++c) has[c] = act && (x0 + c < M);
 
-    // LDS: two staged planes | reduction scratch | the node's projections
+    // LDS: two staged planes | reduction scratch
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
-    double *sv = red + 16 * 4;
     __shared__ double sh_center;
     for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (T)0;
     if (tid == 0) sh_center = 0.0;
-    {
-        const double *__restrict__ sIn = A.nl.s + (size_t)b * A.nl.n_ip;
-        for (int i = tid; i < A.nl.n_ip; i += blockDim.x) sv[i] = sIn[i];
-    }
 
-    // buffer resources of this node's vectors and the shared tables
+    // buffer resources of this node's vectors and of Veff
     const size_t base = (size_t)b * G.M3;
     const size_t vb = (size_t)G.M3 * TS;
     T *Xw = A.X + base;
@@ -207,43 +259,45 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     T *Yob = ((CH && A.O) ? A.O : A.Y) + base;
     T *Db = MODE == MODE_CHEBD ? A.D + base : nullptr;
     const __amdgpu_buffer_rsrc_t rX = brsrc(Xw, vb), rY = brsrc(Yb, vb);
+    const __amdgpu_buffer_rsrc_t rN = brsrc(A.nlv + base, vb);
     const __amdgpu_buffer_rsrc_t rD = brsrc(MODE == MODE_CHEBD ? Db : Yb, vb);
     const __amdgpu_buffer_rsrc_t rV = brsrc(A.veff, (size_t)n * n * n * TS);
-    const int E3 = (int)A.nl.E3;
-    const int KA = A.nl.k_alloc;
-    const __amdgpu_buffer_rsrc_t rEU = brsrc(A.nl.u, (size_t)KA * E3 * TS);
-    const __amdgpu_buffer_rsrc_t rEI = brsrc(A.nl.ip, (size_t)KA * E3 * 4);
-    const __amdgpu_buffer_rsrc_t rEC = brsrc(A.nl.cnt, (size_t)E3);
 
     const int col = py * M + x0;                 // offset of the first column in a plane
     int gx[CPT];
 #pragma unroll
     for (int c = 0; c < CPT; ++c) gx[c] = wrapn(ni - nloc + x0 + c, n);
     const int gyo = wrapn(nj - nloc + py, n) * n;
-    const int EE = E * E;
-    const int ge0 = (nk * E + (nj + py)) * E + (ni + x0);   // extended-grid point at z = 0
-    const double hl = -0.5;
-    const T c0d = (T)(3 * G.lap[0]);
-    T cl[FDN + 1];
+    // Arithmetic is FP64 in both modes: the mixed mode stores vectors, Veff
+    // and the nonlocal term in FP32 (half the HBM traffic) but forms H v, the
+    // recurrence and the scalings in FP64, so that only the storage rounding of
+    // each vector (not ~40 FP32 roundings of terms of size (chi + zee)|t| per
+    // point) enters the moments.
+    typedef double R;
+    const R hl = -0.5;
+    const R c0d = (R)(3 * G.lap[0]);
+    R cl[FDN + 1];
 #pragma unroll
-    for (int a = 0; a <= FDN; ++a) cl[a] = (T)G.lap[a];
+    for (int a = 0; a <= FDN; ++a) cl[a] = (R)G.lap[a];
 
     // per-node scalars
-    T chi = 0, izee = 0, two_izee = 0, ca = 0, ib = 0, bp = 0;
-    T dk = 0, d0 = 0;   // DM-column weights of t_k (and t_0 on the first step)
+    R chi = 0, izee = 0, two_izee = 0;
+    T ca = 0, ib = 0;
+    R bp = 0;
+    R dk = 0, d0 = 0;   // DM-column weights of t_k (and t_0 on the first step)
     if (CH) {
         const int ci_idx = A.chi_by_node ? node : b;
-        chi = (T)A.chi[ci_idx];
-        izee = (T)(1.0 / A.zee[ci_idx]);
-        two_izee = (T)(2.0 / A.zee[ci_idx]);
+        chi = A.chi[ci_idx];
+        izee = 1.0 / A.zee[ci_idx];
+        two_izee = 2.0 / A.zee[ci_idx];
         if (MODE == MODE_CHEBD) {
             const double *ci = A.Ci + (size_t)node * (A.npl + 1);
-            dk = (T)(ci[A.k] * A.inv_h3);
-            d0 = (T)(ci[0] * A.inv_h3);
+            dk = ci[A.k] * A.inv_h3;
+            d0 = ci[0] * A.inv_h3;
         }
     }
     if (MODE == MODE_LANCZOS) {
-        ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = (T)A.bprev[b];
+        ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = A.bprev[b];
     }
     const bool cheb_prev = CH && !A.first;
     const bool dacc = MODE == MODE_CHEBD && !A.first;
@@ -258,15 +312,8 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
         if (MODE == MODE_LANCZOS) return (bload<T>(rX, o) - ca * bload<T>(rY, o)) * ib;
         return bload<T>(rX, o);
     };
-
-    // pointwise inputs of one plane and the ELL count of the plane after;
-    // two sets alternate between the steps of the unrolled loop
     struct PW {
-        T ve[CPT], pv[CPT], dv[CPT];   // Veff, t_{k-1} / v_{k-1}, D
-        int cn[CPT];
-    };
-    auto ecount = [&](int q, int c) -> int {
-        return bload_u8(rEC, (has[c] && q < M) ? (unsigned)(ge0 + q * EE + c) : OOB);
+        T ve[CPT], nl[CPT], pv[CPT], dv[CPT];   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
     };
     auto fill_pw = [&](int z, PW &P) {
 #pragma unroll
@@ -274,66 +321,29 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
             const unsigned o = voff(z, c);
             const int vo = (wrapn(nk - nloc + z, n) * n) * n + gyo + gx[c];
             P.ve[c] = bload<T>(rV, (has[c] && z < M) ? (unsigned)vo * TS : OOB);
+            P.nl[c] = bload<T>(rN, o);
             P.pv[c] = (cheb_prev || MODE == MODE_LANCZOS) ? bload<T>(rY, o) : (T)0;
             P.dv[c] = dacc ? bload<T>(rD, o) : (T)0;
-            P.cn[c] = ecount(z + 1, c);
         }
-    };
-    // ELL entries of plane z (count cz) and the nonlocal value they give
-    T eu[CPT][KS];
-    int eip[CPT][KS];
-    auto load_ell = [&](int z, const int (&cz)[CPT]) {
-#pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-            const int g = ge0 + z * EE + c;
-#pragma unroll
-            for (int j = 0; j < KS; ++j) {
-                const bool ok = K > 0 && j < cz[c];
-                eu[c][j] = bload<T>(rEU, ok ? (unsigned)(j * E3 + g) * TS : OOB);
-                eip[c][j] = bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * 4u : OOB);
-            }
-        }
-    };
-    auto ell_value = [&](int z, int c, int cz) -> T {
-        T a = 0;
-        if (K > 0) {
-#pragma unroll
-            for (int j = 0; j < KS; ++j) a += (T)sv[eip[c][j]] * eu[c][j];
-        } else {
-            const int g = ge0 + z * EE + c;
-            if (has[c] && z < M)
-                for (int j = 0; j < cz; ++j) a += (T)sv[A.nl.ip[j * E3 + g]] * A.nl.u[j * E3 + g];
-        }
-        return a;
     };
 
     // window: w[c][s] holds plane z - 6 + s during step z
-    T w[CPT][WIN], nl[CPT];
+    T w[CPT][WIN];
     PW SA, SB;
-    {
-        int c0[CPT];
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) {
+    for (int c = 0; c < CPT; ++c) {
 #pragma unroll
-            for (int s = 0; s < FDN; ++s) w[c][s] = 0;
+        for (int s = 0; s < FDN; ++s) w[c][s] = 0;
 #pragma unroll
-            for (int s = 0; s <= FDN; ++s) w[c][FDN + s] = feed(s, c);   // planes 0..6
-            c0[c] = ecount(0, c);
-        }
-        fill_pw(0, SA);
-        load_ell(0, c0);
-#pragma unroll
-        for (int c = 0; c < CPT; ++c) nl[c] = ell_value(0, c, c0[c]);
+        for (int s = 0; s <= FDN; ++s) w[c][FDN + s] = feed(s, c);   // planes 0..6
     }
+    fill_pw(0, SA);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
 
-    // step z: S holds plane z's pointwise inputs and the count of plane z + 1;
-    // N receives those of plane z + 1.  The ELL entries and the window feed
-    // are issued first and consumed after the stencil of the same step.
+    // step z: S holds plane z's pointwise inputs; N receives plane z + 1's.
     auto step = [&](int z, PW &S, PW &N) {
         T qf[CPT];
-        load_ell(z + 1, S.cn);
 #pragma unroll
         for (int c = 0; c < CPT; ++c) qf[c] = feed(z + FDN + 1, c);
         fill_pw(z + 1, N);
@@ -348,51 +358,51 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
 #pragma unroll
             for (int c = 0; c < CPT; ++c) {
                 if (!has[c]) continue;
-                const T cv = w[c][FDN];
-                T t = cv * c0d;
+                const R cv = (R)w[c][FDN];
+                R t = cv * c0d;
 #pragma unroll
                 for (int a = 1; a <= FDN; ++a) {
-                    const T xl = row[c - a], xr = row[c + a];
-                    const T ym = row[c - a * PP], yp = row[c + a * PP];
-                    t += (xl + xr + ym + yp + w[c][FDN - a] + w[c][FDN + a]) * cl[a];
+                    const R xl = (R)row[c - a], xr = (R)row[c + a];
+                    const R ym = (R)row[c - a * PP], yp = (R)row[c + a * PP];
+                    t += (xl + xr + ym + yp + (R)w[c][FDN - a] + (R)w[c][FDN + a]) * cl[a];
                     // keep the LDS reads of one offset in flight at a time (register pressure)
-                    if (sizeof(T) == 8) __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                const T hv = (T)hl * t + S.ve[c] * cv + nl[c];
+                const R hv = hl * t + (R)S.ve[c] * cv + (R)S.nl[c];
                 const int ozp = z * M2 + col + c;
                 if (MODE == MODE_HSUB) {
-                    Yb[ozp] = hv;
+                    Yb[ozp] = (T)hv;
                 } else if (CH) {
-                    T o;
+                    R o;
                     if (A.first) o = (hv - chi * cv) * izee;
-                    else o = (hv - chi * cv) * two_izee - S.pv[c];
-                    Yob[ozp] = o;
-                    acc[0] += (double)o * (double)o;
-                    acc[1] += (double)o * (double)cv;
+                    else o = (hv - chi * cv) * two_izee - (R)S.pv[c];
+                    const T os = (T)o;   // the stored value is the one reduced and projected
+                    Yob[ozp] = os;
+                    acc[0] += (double)os * (double)os;
+                    acc[1] += (double)os * cv;
                     if (MODE == MODE_CHEBD) {
                         if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
                             const bool ctr = z == nloc && py == nloc && x0 + c == nloc;
-                            Db[ozp] = o * dk + (ctr ? d0 : (T)0);
+                            Db[ozp] = (T)((R)os * dk + (ctr ? d0 : 0.0));
                         } else {
-                            Db[ozp] = S.dv[c] + o * dk;
+                            Db[ozp] = (T)((R)S.dv[c] + (R)os * dk);
                         }
                     }
-                    if (z == nloc && py == nloc && x0 + c == nloc) sh_center = (double)o;
+                    if (z == nloc && py == nloc && x0 + c == nloc) sh_center = (double)os;
                 } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
-                    const T q = hv - bp * S.pv[c];
+                    const T q = (T)(hv - bp * (R)S.pv[c]);
                     Xw[ozp] = q;
-                    Yb[ozp] = cv;
-                    acc[0] += (double)cv * (double)hv;
+                    Yb[ozp] = (T)cv;
+                    acc[0] += cv * hv;
                     acc[1] += (double)q * (double)q;
-                    acc[2] += (double)q * (double)cv;
-                    acc[3] += (double)cv * (double)cv;
+                    acc[2] += (double)q * cv;
+                    acc[3] += cv * cv;
                 }
             }
         }
-        // nonlocal value of plane z + 1, then shift the window and insert plane z + 7
+        // shift the window and insert plane z + 7
 #pragma unroll
         for (int c = 0; c < CPT; ++c) {
-            nl[c] = ell_value(z + 1, c, S.cn[c]);
 #pragma unroll
             for (int s = 0; s < WIN - 1; ++s) w[c][s] = w[c][s + 1];
             w[c][WIN - 1] = qf[c];
@@ -431,6 +441,7 @@ struct PArgs {
     const int *nlocc, *nproj;     // per type
     const double *inv_denom;      // [type][16]
     const double *ca, *ib;        // LANCZOS: projects v = (V - ca V2) * ib
+    int *overflow;
 };
 
 // s[b][ip] = <U_ip, v_b>/Denom over the intersection of the image's rc-cube
@@ -491,6 +502,137 @@ __global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
             if (i < np) {
                 const double r = wave_sum(acc[i]);
                 if (lane == 0) sb[A.ip_base[J] + i] = r * A.inv_denom[16 * t + i];
+            }
+        }
+    }
+}
+
+// Projections of 16 node vectors at a time on MFMA.  For every image J whose
+// rc-cube meets one of the 16 node cubes, S[i][m] = sum over J's box points x
+// of U_J,i(x) v_m(x) (v_m = 0 outside node m's cube) is the product of the
+// (np x Q) projector block and the (Q x 16) block of the 16 vectors on the
+// box points: FP64 16x16x4 MFMAs with the projectors on the rows (A), the box
+// points on K and the nodes on the columns (B), so that each U value is read
+// once for the 16 nodes.  s_m[J, i] = S[i][m] / Denom_i (nonlocal.cpp:149-171).
+// Operand layout of v_mfma_f64_16x16x4f64 (lane l): A[i][k] with i = l % 16,
+// k = l / 16; B[k][j] with k = l / 16, j = l % 16; D[i][j] in register r with
+// i = 4 r + l / 16, j = l % 16.
+constexpr int PGS = 16;       // nodes per group
+constexpr int GLCAP = 2048;   // images meeting the cubes of one group
+
+template <typename T, int FORMED>
+__global__ void __launch_bounds__(256) k_proj_mfma(PArgs<T> A, int B, int n_img)
+{
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const Geo &G = A.g;
+    const int s0 = blockIdx.x * PGS;
+    const int nmem = min(PGS, B - s0);
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    extern __shared__ unsigned char smem_raw[];
+    const int nwords = (n_img + 31) >> 5;
+    unsigned *bits = reinterpret_cast<unsigned *>(smem_raw);
+    int *glist = reinterpret_cast<int *>(bits + nwords);
+    __shared__ int gcnt, mx[PGS], my[PGS], mz[PGS], mact[PGS], bb[6];
+    __shared__ double mca[PGS], mib[PGS];
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) bits[i] = 0u;
+    if (threadIdx.x < PGS) {
+        const int m = threadIdx.x;
+        int act = 0, nd = 0;
+        double ca = 0.0, ib = 1.0;
+        if (m < nmem) {
+            act = !(G.done && G.done[s0 + m]);
+            nd = G.node_list[s0 + m];
+            if (FORMED) { ca = A.ca[s0 + m]; ib = A.ib[s0 + m]; }
+        }
+        mact[m] = act;
+        mx[m] = nd % n; my[m] = (nd / n) % n; mz[m] = nd / (n * n);
+        mca[m] = ca; mib[m] = ib;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {   // bounding box of the active cubes
+        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+        for (int m = 0; m < nmem; ++m)
+            if (mact[m]) {
+                const int c3[3] = {mx[m], my[m], mz[m]};
+                for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c3[d] - nloc); hi[d] = max(hi[d], c3[d] + nloc); }
+            }
+        for (int d = 0; d < 3; ++d) { bb[d] = lo[d]; bb[3 + d] = hi[d]; }
+    }
+    for (int m = 0; m < nmem; ++m) {   // union of the members' image lists
+        if (!mact[m]) continue;
+        const int cnt = A.list_cnt[s0 + m];
+        for (int li = threadIdx.x; li < cnt; li += blockDim.x) {
+            const int J = A.list[(size_t)(s0 + m) * LCAP + li];
+            atomicOr(&bits[J >> 5], 1u << (J & 31));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int wd = 0; wd < nwords; ++wd) {
+            unsigned v = bits[wd];
+            while (v) {
+                const int bit = __ffs(v) - 1;
+                v &= v - 1;
+                if (c < GLCAP) glist[c] = wd * 32 + bit;
+                ++c;
+            }
+        }
+        if (c > GLCAP) { atomicOr(A.overflow, 1); c = GLCAP; }
+        gcnt = c;
+    }
+    __syncthreads();
+    const int k4 = lane >> 4, col = lane & 15;
+    const bool cact = col < nmem && mact[col];
+    const int cx0 = mx[col] - nloc, cy0 = my[col] - nloc, cz0 = mz[col] - nloc;   // cube origin of node col
+    const T *V = A.V + (size_t)(s0 + (cact ? col : 0)) * G.M3;
+    const T *V2 = FORMED ? A.V2 + (size_t)(s0 + (cact ? col : 0)) * G.M3 : nullptr;
+    const T cam = (T)mca[col], ibm = (T)mib[col];
+    for (int li = wave; li < gcnt; li += nwave) {
+        const int J = glist[li];
+        const int t = A.img_type[J];
+        const int nc = A.nlocc[t], np = A.nproj[t], w = 2 * nc + 1, w3 = w * w * w;
+        const int cx = A.img_nn[3 * J], cy = A.img_nn[3 * J + 1], cz = A.img_nn[3 * J + 2];
+        const int lox = max(cx - nc, bb[0]), hix = min(cx + nc, bb[3]);
+        const int loy = max(cy - nc, bb[1]), hiy = min(cy + nc, bb[4]);
+        const int loz = max(cz - nc, bb[2]), hiz = min(cz + nc, bb[5]);
+        if (lox > hix || loy > hiy || loz > hiz) continue;
+        const T *U = A.udv + A.udv_off[J] + (size_t)(col < np ? col : 0) * w3;
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int z = loz; z <= hiz; ++z)
+            for (int y = loy; y <= hiy; ++y) {
+                const int qrow = ((z - cz + nc) * w + (y - cy + nc)) * w + nc - cx;   // q = qrow + x
+                const int zm = z - cz0, ym = y - cy0;
+                const bool rin = cact && zm >= 0 && zm < M && ym >= 0 && ym < M;
+                const int vrow = zm * M2 + ym * M - cx0;                              // v index = vrow + x
+                // x chunks of 16 anchored at the image box, so that a node's sum
+                // does not depend on which nodes share its group (chunks with
+                // no point of any member cube add exact zeros and are skipped)
+                for (int xb = cx - nc; xb <= cx + nc; xb += 16) {
+                    if (xb + 15 < lox || xb > hix) continue;
+#pragma unroll
+                    for (int mm = 0; mm < 4; ++mm) {
+                        const int x = xb + 4 * k4 + mm;
+                        double a = 0.0, bv = 0.0;
+                        if (x <= cx + nc) {
+                            if (col < np) a = (double)U[qrow + x];
+                            const int xm = x - cx0;
+                            if (rin && xm >= 0 && xm < M) {
+                                if (FORMED) bv = (double)((V[vrow + x] - cam * V2[vrow + x]) * ibm);
+                                else bv = (double)V[vrow + x];
+                            }
+                        }
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+                    }
+                }
+            }
+        if (cact) {
+            double *sb = A.s + (size_t)(s0 + col) * A.n_ip + A.ip_base[J];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * r + k4;
+                if (i < np) sb[i] = acc[r] * A.inv_denom[16 * t + i];
             }
         }
     }
@@ -720,6 +862,37 @@ __device__ int ql_extremes(double *d, double *e, int n, double *lmin, double *lm
     return 0;
 }
 
+// Largest (sign > 0) or smallest (sign < 0) eigenvalue of the symmetric
+// tridiagonal (d[0..n-1], e[0..n-2]) by Newton's method on det(xI - T) from a
+// start x0 outside the spectrum.  With q_i = p_i/p_{i-1} (Sturm ratios) and
+// r_i = q_i'/q_i, p'/p = sum_i q_i'/q_i; past the extreme root the
+// characteristic polynomial is monotone and convex (concave on the left), so
+// the iterates move monotonically onto the root and the loop stops at the
+// first step that no longer moves inward.  Returns false after maxit steps.
+__device__ bool tridiag_extreme(const double *d, const double *e, int n, double x0, int sign, double *root)
+{
+    double x = x0;
+    for (int it = 0; it < 200; ++it) {
+        double q = x - d[0], dq = 1.0;
+        double iq = 1.0 / q;
+        double s = dq * iq;
+        for (int i = 1; i < n; ++i) {
+            const double r = e[i - 1] * e[i - 1] * iq;
+            dq = 1.0 + r * dq * iq;
+            q = (x - d[i]) - r;
+            iq = 1.0 / q;
+            s += dq * iq;
+        }
+        const double xn = x - 1.0 / s;
+        if (!(sign > 0 ? xn < x : xn > x)) {
+            *root = x;
+            return true;
+        }
+        x = xn;
+    }
+    return false;
+}
+
 __global__ void k_lanczos_scalars(QArgs A)
 {
 #pragma clang fp contract(off)
@@ -739,10 +912,29 @@ __global__ void k_lanczos_scalars(QArgs A)
     A.bprev[b] = be;
     if (p == 0) return;
     const int nq = p + 1;
-    double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
-    for (int i = 0; i < nq; ++i) { d[i] = A.aa[(size_t)b * S + i]; e[i] = A.bb[(size_t)b * S + i]; }
-    double lo, hi;
-    if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
+    // Extreme eigenvalues of the (p+1)-sized tridiagonal.  The reference runs
+    // the full QL (TridiagEigenSolve, sq.cpp:1094-1180) every pass; only the
+    // extremes are used, and adding row p moves them by at most |beta_{p-1}|
+    // beyond max(lambda_max(T_p), alpha_p) (min for lambda_min), which gives
+    // Newton a start outside the spectrum.  QL stays as the fallback.
+    const double *ad = A.aa + (size_t)b * S, *be2 = A.bb + (size_t)b * S;
+    double lo, hi, up, dn;
+    if (p == 1) {   // Gershgorin
+        const double r = fabs(be2[0]);
+        up = fmax(ad[0], ad[1]) + r;
+        dn = fmin(ad[0], ad[1]) - r;
+    } else {
+        const double r = fabs(be2[p - 1]);
+        up = fmax(A.lmax_prev[b], ad[p]) + r;
+        dn = fmin(A.lmin_prev[b], ad[p]) - r;
+    }
+    up += 8.0 * 2.2e-16 * fabs(up) + 1e-300;
+    dn -= 8.0 * 2.2e-16 * fabs(dn) + 1e-300;
+    if (!tridiag_extreme(ad, be2, nq, up, 1, &hi) || !tridiag_extreme(ad, be2, nq, dn, -1, &lo)) {
+        double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
+        for (int i = 0; i < nq; ++i) { d[i] = ad[i]; e[i] = be2[i]; }
+        if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
+    }
     const double dl = fabs(lo - A.lmin_prev[b]), dm = fabs(hi - A.lmax_prev[b]);
     A.lmin_prev[b] = lo;
     A.lmax_prev[b] = hi;

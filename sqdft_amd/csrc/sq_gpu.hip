@@ -139,10 +139,9 @@ ZShape zmarch_shape(int M, bool fp64)
 }
 int zmarch_threads(int M, bool fp64) { return zmarch_threads_cpt(M, zmarch_shape(M, fp64).cpt); }
 
-// Visiting order of the FD nodes: 4x4x2 bricks, so that the 16 consecutive
-// batch slots that one projection workgroup takes (PGS) are a compact 4x4 slab
-// whose cubes share nearly all their images, and neighbouring workgroups of
-// the z-march and of the nonlocal apply touch overlapping cubes.
+// Visiting order of the FD nodes: 4x4x2 bricks, so that workgroups that run
+// at the same time (consecutive batch slots) march overlapping cubes and
+// share Veff, projector tables and ELL entries in the caches.
 std::vector<int> node_order(int n)
 {
     const int bx = 4, by = 4, bz = 2;
@@ -227,7 +226,6 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     A.inv_denom = c->inv_denom.as<double>();
     A.ca = c->ca.as<double>();
     A.ib = c->ib.as<double>();
-    A.overflow = c->flags.as<int>() + 3;
     return A;
 }
 
@@ -246,10 +244,12 @@ template <typename T, int FORMED>
 void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
 {
     if (c->n_ip > 0) {
-        const size_t shm = (size_t)((c->n_img + 31) / 32) * 4 + (size_t)GLCAP * 4;
-        auto fn = k_proj_mfma<T, FORMED>;
-        if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        hipLaunchKernelGGL(fn, dim3((B + PGS - 1) / PGS), dim3(256), shm, c->st, A, B, c->n_img);
+        if (c->max_nproj <= 1)
+            hipLaunchKernelGGL((k_proj<T, 1, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+        else if (c->max_nproj <= 4)
+            hipLaunchKernelGGL((k_proj<T, 4, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+        else
+            hipLaunchKernelGGL((k_proj<T, 16, FORMED>), dim3(B), dim3(256), 0, c->st, A);
     }
     NArgs<T> N{};
     N.g = A.g;

@@ -441,7 +441,6 @@ struct PArgs {
     const int *nlocc, *nproj;     // per type
     const double *inv_denom;      // [type][16]
     const double *ca, *ib;        // LANCZOS: projects v = (V - ca V2) * ib
-    int *overflow;
 };
 
 // s[b][ip] = <U_ip, v_b>/Denom over the intersection of the image's rc-cube
@@ -502,137 +501,6 @@ __global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
             if (i < np) {
                 const double r = wave_sum(acc[i]);
                 if (lane == 0) sb[A.ip_base[J] + i] = r * A.inv_denom[16 * t + i];
-            }
-        }
-    }
-}
-
-// Projections of 16 node vectors at a time on MFMA.  For every image J whose
-// rc-cube meets one of the 16 node cubes, S[i][m] = sum over J's box points x
-// of U_J,i(x) v_m(x) (v_m = 0 outside node m's cube) is the product of the
-// (np x Q) projector block and the (Q x 16) block of the 16 vectors on the
-// box points: FP64 16x16x4 MFMAs with the projectors on the rows (A), the box
-// points on K and the nodes on the columns (B), so that each U value is read
-// once for the 16 nodes.  s_m[J, i] = S[i][m] / Denom_i (nonlocal.cpp:149-171).
-// Operand layout of v_mfma_f64_16x16x4f64 (lane l): A[i][k] with i = l % 16,
-// k = l / 16; B[k][j] with k = l / 16, j = l % 16; D[i][j] in register r with
-// i = 4 r + l / 16, j = l % 16.
-constexpr int PGS = 16;       // nodes per group
-constexpr int GLCAP = 2048;   // images meeting the cubes of one group
-
-template <typename T, int FORMED>
-__global__ void __launch_bounds__(256) k_proj_mfma(PArgs<T> A, int B, int n_img)
-{
-    typedef double d4 __attribute__((ext_vector_type(4)));
-    const Geo &G = A.g;
-    const int s0 = blockIdx.x * PGS;
-    const int nmem = min(PGS, B - s0);
-    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    extern __shared__ unsigned char smem_raw[];
-    const int nwords = (n_img + 31) >> 5;
-    unsigned *bits = reinterpret_cast<unsigned *>(smem_raw);
-    int *glist = reinterpret_cast<int *>(bits + nwords);
-    __shared__ int gcnt, mx[PGS], my[PGS], mz[PGS], mact[PGS], bb[6];
-    __shared__ double mca[PGS], mib[PGS];
-    for (int i = threadIdx.x; i < nwords; i += blockDim.x) bits[i] = 0u;
-    if (threadIdx.x < PGS) {
-        const int m = threadIdx.x;
-        int act = 0, nd = 0;
-        double ca = 0.0, ib = 1.0;
-        if (m < nmem) {
-            act = !(G.done && G.done[s0 + m]);
-            nd = G.node_list[s0 + m];
-            if (FORMED) { ca = A.ca[s0 + m]; ib = A.ib[s0 + m]; }
-        }
-        mact[m] = act;
-        mx[m] = nd % n; my[m] = (nd / n) % n; mz[m] = nd / (n * n);
-        mca[m] = ca; mib[m] = ib;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {   // bounding box of the active cubes
-        int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-        for (int m = 0; m < nmem; ++m)
-            if (mact[m]) {
-                const int c3[3] = {mx[m], my[m], mz[m]};
-                for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c3[d] - nloc); hi[d] = max(hi[d], c3[d] + nloc); }
-            }
-        for (int d = 0; d < 3; ++d) { bb[d] = lo[d]; bb[3 + d] = hi[d]; }
-    }
-    for (int m = 0; m < nmem; ++m) {   // union of the members' image lists
-        if (!mact[m]) continue;
-        const int cnt = A.list_cnt[s0 + m];
-        for (int li = threadIdx.x; li < cnt; li += blockDim.x) {
-            const int J = A.list[(size_t)(s0 + m) * LCAP + li];
-            atomicOr(&bits[J >> 5], 1u << (J & 31));
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int c = 0;
-        for (int wd = 0; wd < nwords; ++wd) {
-            unsigned v = bits[wd];
-            while (v) {
-                const int bit = __ffs(v) - 1;
-                v &= v - 1;
-                if (c < GLCAP) glist[c] = wd * 32 + bit;
-                ++c;
-            }
-        }
-        if (c > GLCAP) { atomicOr(A.overflow, 1); c = GLCAP; }
-        gcnt = c;
-    }
-    __syncthreads();
-    const int k4 = lane >> 4, col = lane & 15;
-    const bool cact = col < nmem && mact[col];
-    const int cx0 = mx[col] - nloc, cy0 = my[col] - nloc, cz0 = mz[col] - nloc;   // cube origin of node col
-    const T *V = A.V + (size_t)(s0 + (cact ? col : 0)) * G.M3;
-    const T *V2 = FORMED ? A.V2 + (size_t)(s0 + (cact ? col : 0)) * G.M3 : nullptr;
-    const T cam = (T)mca[col], ibm = (T)mib[col];
-    for (int li = wave; li < gcnt; li += nwave) {
-        const int J = glist[li];
-        const int t = A.img_type[J];
-        const int nc = A.nlocc[t], np = A.nproj[t], w = 2 * nc + 1, w3 = w * w * w;
-        const int cx = A.img_nn[3 * J], cy = A.img_nn[3 * J + 1], cz = A.img_nn[3 * J + 2];
-        const int lox = max(cx - nc, bb[0]), hix = min(cx + nc, bb[3]);
-        const int loy = max(cy - nc, bb[1]), hiy = min(cy + nc, bb[4]);
-        const int loz = max(cz - nc, bb[2]), hiz = min(cz + nc, bb[5]);
-        if (lox > hix || loy > hiy || loz > hiz) continue;
-        const T *U = A.udv + A.udv_off[J] + (size_t)(col < np ? col : 0) * w3;
-        d4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int z = loz; z <= hiz; ++z)
-            for (int y = loy; y <= hiy; ++y) {
-                const int qrow = ((z - cz + nc) * w + (y - cy + nc)) * w + nc - cx;   // q = qrow + x
-                const int zm = z - cz0, ym = y - cy0;
-                const bool rin = cact && zm >= 0 && zm < M && ym >= 0 && ym < M;
-                const int vrow = zm * M2 + ym * M - cx0;                              // v index = vrow + x
-                // x chunks of 16 anchored at the image box, so that a node's sum
-                // does not depend on which nodes share its group (chunks with
-                // no point of any member cube add exact zeros and are skipped)
-                for (int xb = cx - nc; xb <= cx + nc; xb += 16) {
-                    if (xb + 15 < lox || xb > hix) continue;
-#pragma unroll
-                    for (int mm = 0; mm < 4; ++mm) {
-                        const int x = xb + 4 * k4 + mm;
-                        double a = 0.0, bv = 0.0;
-                        if (x <= cx + nc) {
-                            if (col < np) a = (double)U[qrow + x];
-                            const int xm = x - cx0;
-                            if (rin && xm >= 0 && xm < M) {
-                                if (FORMED) bv = (double)((V[vrow + x] - cam * V2[vrow + x]) * ibm);
-                                else bv = (double)V[vrow + x];
-                            }
-                        }
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
-                    }
-                }
-            }
-        if (cact) {
-            double *sb = A.s + (size_t)(s0 + col) * A.n_ip + A.ip_base[J];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 4 * r + k4;
-                if (i < np) sb[i] = acc[r] * A.inv_denom[16 * t + i];
             }
         }
     }

@@ -633,16 +633,25 @@ __global__ void k_node_images(LArgs A, int B)
     const int n = A.g.n, nloc = A.g.nloc;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
     int cnt = 0;
+    int *L = A.list + (size_t)b * LCAP;
     for (int J = 0; J < A.n_img; ++J) {
         const int t = A.img_type[J];
         if (A.nproj[t] == 0) continue;
         const int R = nloc + A.nlocc[t];
         const int dx = A.img_nn[3 * J] - ni, dy = A.img_nn[3 * J + 1] - nj, dz = A.img_nn[3 * J + 2] - nk;
         if (dx < -R || dx > R || dy < -R || dy > R || dz < -R || dz > R) continue;
-        if (cnt < LCAP) A.list[(size_t)b * LCAP + cnt] = J;
+        if (cnt < LCAP) L[cnt] = J;
         ++cnt;
     }
     if (cnt > LCAP) { atomicOr(A.overflow, 1); cnt = LCAP; }
+    // stable insertion sort by the image's z (then list order): the waves of
+    // the projections then sweep the cube in z and share its planes in cache
+    for (int i = 1; i < cnt; ++i) {
+        const int J = L[i], kz = A.img_nn[3 * J + 2];
+        int j = i - 1;
+        while (j >= 0 && A.img_nn[3 * L[j] + 2] > kz) { L[j + 1] = L[j]; --j; }
+        L[j + 1] = J;
+    }
     A.list_cnt[b] = cnt;
 }
 

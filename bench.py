@@ -66,6 +66,41 @@ def cpu_baseline(system, veff, Rcut, T, seconds, seed=5):
             "seconds": dt}
 
 
+def profiled_traffic(workload: str, precision: str) -> dict:
+    """HBM bytes per dispatch of the z-march kernels as measured by the counter
+    runs recorded under profiles/ (lines `traffic <workload> <precision>
+    <kernel> <bytes>`); empty when no run covers this workload."""
+    out = {}
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return out
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                t = line.split()
+                if len(t) == 5 and t[0] == "traffic" and t[1] == workload and t[2] == precision:
+                    out[t[3]] = float(t[4])
+    return out
+
+
+def calibration_copy(nb: int) -> None:
+    """One device-to-device copy of nb bytes through the HIP runtime (the
+    runtime's copy kernel): nb bytes read and nb written, the reference that
+    scales the FETCH_SIZE / WRITE_SIZE counters of the other kernels."""
+    import ctypes as C
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    a, b = C.c_void_p(), C.c_void_p()
+    for call in (lambda: hip.hipMalloc(C.byref(a), C.c_size_t(nb)), lambda: hip.hipMalloc(C.byref(b), C.c_size_t(nb)),
+                 lambda: hip.hipMemset(a, 1, C.c_size_t(nb)), lambda: hip.hipDeviceSynchronize(),
+                 lambda: hip.hipMemcpy(b, a, C.c_size_t(nb), 3), lambda: hip.hipDeviceSynchronize(),
+                 lambda: hip.hipFree(a), lambda: hip.hipFree(b)):
+        if call() != 0:
+            raise RuntimeError("calibration copy: a HIP runtime call failed")
+    log(f"bench: calibration copy of {nb} bytes read + {nb} bytes written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,15 +133,7 @@ def main():
     info = S.info
     nn = S.nn
     if args.copy_calib_mib > 0:
-        import torch
-        nb = args.copy_calib_mib * (1 << 20)
-        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"bench: calibration copy of {nb} bytes read + {nb} bytes written")
-        del src, dst
+        calibration_copy(args.copy_calib_mib * (1 << 20))
     for i in range(args.warmup):
         log(f"bench: warmup {i}")
         sq.density(veff, scf_iter=1)
@@ -121,20 +148,29 @@ def main():
     st = last["stats"]
     # dominant z-march kernel: HIP events around every launch of the last step
     kt = [sq.kernel_timing(0), sq.kernel_timing(1)]
-    names = ["k_zmarch<CHEB> (Chebyshev recurrence)", "k_zmarch<LANCZOS> (Lanczos pass)"]
+    keys = ["k_zmarch<CHEB>", "k_zmarch<LANCZOS>"]
+    what = ["Chebyshev recurrence: read t_k, t_{k-1}, write t_{k+1}",
+            "Lanczos pass: read r', v_{k-1}, write v_k, r'_{k+1}"]
+    nvec = [3, 4]
     dom = int(np.argmax([k["total_ms"] for k in kt]))
     rs = 8 if args.precision == "fp64" else 4
+    N = (2 * info["nloc"] + 1) ** 3
+    traffic = profiled_traffic(args.workload, args.precision)
     rl = {}
     for w in (0, 1):
         k = kt[w]
         avg_ms = k["total_ms"] / max(k["launches"], 1)
         alg = k["alg_bytes"] / max(k["launches"], 1)
-        rl[w] = {"kernel": names[w], "launches": k["launches"], "avg_ms": avg_ms,
-                 "alg_bytes_per_launch": alg, "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0}
+        rl[w] = {"kernel": f"{keys[w]} ({what[w]})", "launches": k["launches"], "avg_ms": avg_ms,
+                 "alg_bytes_per_launch": alg, "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0,
+                 "alg_bytes_per_unit": f"{nvec[w]} * {N} * {rs} B per node-step",
+                 "traffic": traffic.get(keys[w])}
     roof = {"bound": "hbm", "kernel": rl[dom]["kernel"], "achieved": rl[dom]["achieved_GBs"],
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rl[dom]["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": None,
-            "alg_bytes_per_unit": f"3 * {(2 * info['nloc'] + 1) ** 3} * {rs} B per node-step",
+            "traffic": rl[dom]["traffic"],
+            "alg_bytes_per_unit": rl[dom]["alg_bytes_per_unit"],
+            "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a plain copy "
+                              "(profiles/zmarch_traffic_lih216.md)" if rl[dom]["traffic"] else None,
             "other": rl[1 - dom]}
     out = {"metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",

@@ -74,7 +74,10 @@ typedef struct sqg_params {
 } sqg_params;
 
 typedef struct sqg_stats {
-    double t_lanczos, t_cheb, t_fermi, t_rho, t_total;   /* seconds, GPU events */
+    /* seconds, GPU events: t_lanczos = every Lanczos pass (projections, nonlocal
+       apply, z-march, eigenvalue bounds) summed over batches; t_cheb = the rest
+       of the moments phase (image lists, Chebyshev recurrence, moments) */
+    double t_lanczos, t_cheb, t_fermi, t_rho, t_total;
     int lanczos_iters_node0;  /* the count the reference prints for node 0 */
     int lanczos_iters_min, lanczos_iters_max;
     long lanczos_passes;      /* Hsub passes spent in Lanczos, summed over nodes */

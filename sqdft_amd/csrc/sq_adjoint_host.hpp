@@ -28,18 +28,8 @@ int batch_images(sqg_ctx *c, int B)
 {
     const int *dn = c->node_list.as<int>();
     HIPCHK(hipMemsetAsync(c->flags.p, 0, 16 * sizeof(int), c->st));
-    LArgs L{};
-    L.g = make_geo(c, dn, nullptr);
-    L.n_img = c->n_img;
-    L.img_type = c->img_type.as<int>();
-    L.img_nn = c->img_nn.as<int>();
-    L.nlocc = c->nlocc.as<int>();
-    L.nproj = c->nproj.as<int>();
-    L.list = c->list.as<int>();
-    L.list_cnt = c->list_cnt.as<int>();
-    L.overflow = c->flags.as<int>();
-    hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
-    return SQG_OK;
+    (void)dn;
+    return node_images(c, B);
 }
 
 // Sign of rho_pc and d rho_pc/d lambda per node at the last lambda_f

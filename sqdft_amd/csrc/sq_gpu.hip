@@ -73,6 +73,8 @@ struct sqg_ctx {
     // batch buffers
     int Bcap = 0;
     DBuf X, Y, NLv, s, node_list, list, list_cnt, flags;
+    DBuf glist, glist_cnt;         // group image lists (NG slots per group)
+    int gl_max = 0;                // longest group list of the current batch
     DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, cnt, done, ca, ib, bprev;
     DBuf chiB, zeeB, center, dots, htemp;
     // all-node state of the last density step
@@ -83,12 +85,13 @@ struct sqg_ctx {
     DBuf adj, adj_sgn, adj_dl, adj_wt, adj_in, adj_out, adj_rec;
     bool have_state = false;
     double lambda_f = 0.0;
-    // timing of the dominant kernels (which: 0 Chebyshev z-march, 1 Lanczos z-march)
-    std::vector<EventPair> ev[2];
+    // timing of the dominant kernels (which: 0 Chebyshev z-march, 1 Lanczos
+    // z-march) and of the whole Lanczos phase of each batch (2)
+    std::vector<EventPair> ev[3];
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
-    long last_launches[2] = {0, 0};
-    double last_ms[2] = {0, 0}, last_bytes[2] = {0, 0};
+    long last_launches[3] = {0, 0, 0};
+    double last_ms[3] = {0, 0, 0}, last_bytes[3] = {0, 0, 0};
 
     size_t real_size() const { return p.precision == SQG_MIXED ? sizeof(float) : sizeof(double); }
     ~sqg_ctx()
@@ -169,6 +172,8 @@ int ensure_batch(sqg_ctx *c, int B)
     HIPCHK(c->node_list.ensure((size_t)B * sizeof(int)));
     HIPCHK(c->list.ensure((size_t)B * LCAP * sizeof(int)));
     HIPCHK(c->list_cnt.ensure((size_t)B * sizeof(int)));
+    HIPCHK(c->glist.ensure((size_t)((B + NG - 1) / NG) * GLCAP * sizeof(int)));
+    HIPCHK(c->glist_cnt.ensure((size_t)((B + NG - 1) / NG) * sizeof(int)));
     HIPCHK(c->flags.ensure(16 * sizeof(int)));
     HIPCHK(c->lred.ensure((size_t)B * 4 * sizeof(double)));
     for (DBuf *d : {&c->aa, &c->bb, &c->dq, &c->eq}) HIPCHK(d->ensure((size_t)B * S * sizeof(double)));
@@ -189,7 +194,7 @@ int choose_batch(sqg_ctx *c, int want, int nvec = 2)
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
         const size_t per = (nvec + 1) * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
-                           LCAP * 4 + 4 * (MAXIT + 2) * 8 + 256;
+                           LCAP * 4 + GLCAP * 4 / NG + 4 * (MAXIT + 2) * 8 + 256;
         const size_t cap = (size_t)(0.6 * (double)(freeb + (size_t)c->Bcap * per)) / per;
         if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
     }
@@ -216,6 +221,8 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     A.n_ip = std::max(c->n_ip, 1);
     A.list = c->list.as<int>();
     A.list_cnt = c->list_cnt.as<int>();
+    A.glist = c->glist.as<int>();
+    A.glist_cnt = c->glist_cnt.as<int>();
     A.img_type = c->img_type.as<int>();
     A.img_nn = c->img_nn.as<int>();
     A.ip_base = c->ip_base.as<int>();
@@ -229,13 +236,67 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     return A;
 }
 
+// Image lists of the B nodes in node_list (per node and per group of NG
+// slots); sets gl_max.  Overflow goes to flags[0].
+int node_images(sqg_ctx *c, int B)
+{
+    LArgs L{};
+    L.g = make_geo(c, c->node_list.as<int>(), nullptr);
+    L.n_img = c->n_img;
+    L.img_type = c->img_type.as<int>();
+    L.img_nn = c->img_nn.as<int>();
+    L.nlocc = c->nlocc.as<int>();
+    L.nproj = c->nproj.as<int>();
+    L.list = c->list.as<int>();
+    L.list_cnt = c->list_cnt.as<int>();
+    L.overflow = c->flags.as<int>();
+    L.glist = c->glist.as<int>();
+    L.glist_cnt = c->glist_cnt.as<int>();
+    hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+    const int ng = (B + NG - 1) / NG;
+    hipLaunchKernelGGL(k_group_images, dim3((ng + 63) / 64), dim3(64), 0, c->st, L, B);
+    std::vector<int> gc(ng);
+    HIPCHK(hipMemcpyAsync(gc.data(), c->glist_cnt.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->gl_max = 0;
+    for (int v : gc) c->gl_max = std::max(c->gl_max, v);
+    return SQG_OK;
+}
+
+// LDS bytes of one k_nlapplyg workgroup (loc table, row offsets, NG values per row)
+size_t nlapplyg_lds(const sqg_ctx *c)
+{
+    const size_t head = ((size_t)(std::max(c->n_ip, 1) + c->gl_max + 1) * 4 + 15) / 16 * 16;
+    return head + (size_t)std::max(c->gl_max * c->max_nproj, 1) * NG * sizeof(double);
+}
+
 template <typename T, int K>
 void nlapply_go(sqg_ctx *c, int B, const NArgs<T> &A)
 {
+    const size_t shg = nlapplyg_lds(c);
+    if (shg <= 160 * 1024) {
+        NGArgs<T> N{};
+        N.g = A.g; N.nl = A.nl; N.nlv = A.nlv;
+        N.glist = c->glist.as<int>(); N.glist_cnt = c->glist_cnt.as<int>();
+        N.img_type = c->img_type.as<int>(); N.nproj = c->nproj.as<int>(); N.ip_base = c->ip_base.as<int>();
+        N.B = B;
+        auto fn = k_nlapplyg<T>;
+        if (shg > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shg);
+        hipLaunchKernelGGL(fn, dim3((B + NG - 1) / NG), dim3(512), shg, c->st, N);
+        return;
+    }
     const size_t shm = (size_t)std::max(c->n_ip, 1) * sizeof(double);
     auto fn = k_nlapply<T, K>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     hipLaunchKernelGGL(fn, dim3(B), dim3(512), shm, c->st, A);
+}
+
+template <typename T, int MAXP, int FORMED>
+void projs_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
+{
+    auto fn = k_projs<T, MAXP, FORMED>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(fn, grid, dim3(projs_threads<MAXP>()), shm, c->st, P);
 }
 
 // Projections s = <U, v>/Denom of the B vectors (k_proj), then their
@@ -244,13 +305,33 @@ template <typename T, int FORMED>
 void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
 {
     if (c->n_ip > 0) {
+        PArgs<T> P = A;
+        P.B = B;
+        const dim3 grid((B + NG - 1) / NG);
+        // staged projections (k_projs) when the per-image totals and a union
+        // plane of a 4x4 footprint (else a single cube plane) fit in LDS
+        const int mp = c->max_nproj <= 1 ? 1 : c->max_nproj <= 4 ? 4 : 0;
+        if (mp) {
+            const size_t sa = (size_t)std::max(c->gl_max, 1) * (NG * mp * sizeof(double) + sizeof(ImgMeta));
+            for (int pu : {(c->M + 3) * (c->M + 3), c->M2}) {
+                const size_t shs = sa + (size_t)pu * NG * sizeof(T);
+                if (shs > 160 * 1024) continue;
+                P.pu = pu;
+                if (mp == 1) projs_go<T, 1, FORMED>(c, grid, shs, P);
+                else projs_go<T, 4, FORMED>(c, grid, shs, P);
+                goto apply;
+            }
+        }
         if (c->max_nproj <= 1)
-            hipLaunchKernelGGL((k_proj<T, 1, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+            hipLaunchKernelGGL((k_projg<T, 1, FORMED>), grid, dim3(256), 0, c->st, P);
         else if (c->max_nproj <= 4)
-            hipLaunchKernelGGL((k_proj<T, 4, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+            hipLaunchKernelGGL((k_projg<T, 4, FORMED>), grid, dim3(256), 0, c->st, P);
+        else if (c->max_nproj <= 8)
+            hipLaunchKernelGGL((k_projg<T, 8, FORMED>), grid, dim3(256), 0, c->st, P);
         else
-            hipLaunchKernelGGL((k_proj<T, 16, FORMED>), dim3(B), dim3(256), 0, c->st, A);
+            hipLaunchKernelGGL((k_projg<T, 16, FORMED>), grid, dim3(256), 0, c->st, P);
     }
+apply:
     NArgs<T> N{};
     N.g = A.g;
     N.nl.cnt = c->ell_cnt.as<unsigned char>();
@@ -317,6 +398,8 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     const size_t vlen = (size_t)B * c->M3;
     T *X = c->X.as<T>(), *Y = c->Y.as<T>();
     const double rs = (double)sizeof(T);
+    hipEvent_t ph0 = c->ev_get(), ph1 = c->ev_get();
+    if (ph0) (void)hipEventRecord(ph0, c->st);
     // ---------------- Lanczos (sq.cpp:965-1092)
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, vlen, (T)1);
     hipLaunchKernelGGL((k_fill<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, Y, vlen, (T)0);
@@ -354,7 +437,8 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
         Z.X = X; Z.Y = Y;
         Z.ca = c->ca.as<double>(); Z.ib = c->ib.as<double>(); Z.bprev = c->bprev.as<double>();
         Z.lred = c->lred.as<double>();
-        launch_zmarch<T, MODE_LANCZOS>(c, B, Z, 1, 3.0 * active * c->M3 * rs);
+        // reads r' and v_{k-1}, writes v_k and r'_{k+1}
+        launch_zmarch<T, MODE_LANCZOS>(c, B, Z, 1, 4.0 * active * c->M3 * rs);
         passes += active;
         Q.pass = pass;
         hipLaunchKernelGGL(k_lanczos_scalars, dim3((B + 127) / 128), dim3(128), 0, c->st, Q);
@@ -366,6 +450,10 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
         }
     }
     if (passes_out) *passes_out += passes;
+    if (ph0 && ph1) {
+        (void)hipEventRecord(ph1, c->st);
+        c->ev[2].push_back({ph0, ph1, 0.0});
+    }
     if (std::getenv("SQDFT_DEBUG_LANCZOS")) {
         // diagnostic: alpha/beta of the first node of the batch
         std::vector<double> ha(MAXIT + 2), hb(MAXIT + 2);
@@ -387,17 +475,8 @@ int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
     HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
     // image lists
     {
-        LArgs L{};
-        L.g = make_geo(c, dn, nullptr);
-        L.n_img = c->n_img;
-        L.img_type = c->img_type.as<int>();
-        L.img_nn = c->img_nn.as<int>();
-        L.nlocc = c->nlocc.as<int>();
-        L.nproj = c->nproj.as<int>();
-        L.list = c->list.as<int>();
-        L.list_cnt = c->list_cnt.as<int>();
-        L.overflow = flags + 0;
-        hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+        const int st = node_images(c, B);
+        if (st) return st;
     }
     const size_t vlen = (size_t)B * c->M3;
     T *X = c->X.as<T>(), *Y = c->Y.as<T>();
@@ -626,13 +705,10 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
     const int *dn = c->node_list.as<int>();
     int *flags = c->flags.as<int>();
     HIPCHK(hipMemsetAsync(flags, 0, 16 * sizeof(int), c->st));
-    LArgs L{};
-    L.g = make_geo(c, dn, nullptr);
-    L.n_img = c->n_img;
-    L.img_type = c->img_type.as<int>(); L.img_nn = c->img_nn.as<int>();
-    L.nlocc = c->nlocc.as<int>(); L.nproj = c->nproj.as<int>();
-    L.list = c->list.as<int>(); L.list_cnt = c->list_cnt.as<int>(); L.overflow = flags + 0;
-    hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
+    {
+        const int st = node_images(c, B);
+        if (st) return st;
+    }
     const size_t vlen = (size_t)B * c->M3;
     HIPCHK(c->Dm.ensure(vlen * sizeof(T)));
     HIPCHK(c->contrib.ensure((size_t)B * FSLOT * 4 * sizeof(double)));
@@ -905,12 +981,12 @@ int sqg_node_moments(sqg_ctx *c, const double *veff, const int *nodes, int nb, d
     if (!st) st = ensure_node_state(c);
     if (st) return st;
     c->have_state = false;
-    for (int w = 0; w < 2; ++w) c->ev[w].clear();
+    for (int w = 0; w < 3; ++w) c->ev[w].clear();
     c->pool_used = 0;
     long passes = 0;
     st = moments_dispatch(c, nodes, nb, &passes);
     if (st) return st;
-    for (int w = 0; w < 2; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+    for (int w = 0; w < 3; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
     const size_t K = (size_t)c->p.npl + 1;
     std::vector<double> a(c->nn), z(c->nn);
     std::vector<int> it(c->nn);
@@ -942,7 +1018,7 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     if (!st) st = ensure_node_state(c);
     if (st) return st;
     c->have_state = false;
-    for (int w = 0; w < 2; ++w) c->ev[w].clear();
+    for (int w = 0; w < 3; ++w) c->ev[w].clear();
     c->pool_used = 0;
     HIPCHK(hipEventRecord(t0, c->st));
     const int cap = choose_batch(c, c->nn);
@@ -989,14 +1065,14 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     if (ebs) *ebs = Ebs;
     if (eent) *eent = Eent;
     c->have_state = true;
-    for (int w = 0; w < 2; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+    for (int w = 0; w < 3; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
     if (stats) {
         float a = 0, b = 0, d = 0;
         (void)hipEventElapsedTime(&a, t0, t1);
         (void)hipEventElapsedTime(&b, t1, t2);
         (void)hipEventElapsedTime(&d, t2, t3);
         std::memset(stats, 0, sizeof(*stats));
-        stats->t_lanczos = c->last_ms[1] * 1e-3;
+        stats->t_lanczos = c->last_ms[2] * 1e-3;
         stats->t_cheb = a * 1e-3 - stats->t_lanczos;
         stats->t_fermi = b * 1e-3;
         stats->t_rho = d * 1e-3;
@@ -1046,13 +1122,8 @@ int sqg_hsub_apply_batch(sqg_ctx *c, const double *veff, const int *nodes, int n
     HIPCHK(hipMemcpy(tmp.p, v, len * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(c->flags.p, 0, 16 * sizeof(int), c->st));
     const int *dn = c->node_list.as<int>();
-    LArgs L{};
-    L.g = make_geo(c, dn, nullptr);
-    L.n_img = c->n_img;
-    L.img_type = c->img_type.as<int>(); L.img_nn = c->img_nn.as<int>();
-    L.nlocc = c->nlocc.as<int>(); L.nproj = c->nproj.as<int>();
-    L.list = c->list.as<int>(); L.list_cnt = c->list_cnt.as<int>(); L.overflow = c->flags.as<int>();
-    hipLaunchKernelGGL(k_node_images, dim3((nb + 127) / 128), dim3(128), 0, c->st, L, nb);
+    st = node_images(c, nb);
+    if (st) return st;
     auto run = [&](auto tag) -> int {
         using T = decltype(tag);
         T *X = c->X.as<T>(), *Y = c->Y.as<T>();

@@ -4,6 +4,7 @@
 //   vectors   X, Y      [B][M^3]  M = 2 nloc + 1, each node cube z-major (z, y, x)
 //   projections s       [B][n_ip] FP64, one value per image-projector
 //   image lists         [B][LCAP] image ids whose rc-cube meets the node cube
+//   group image lists   [ceil(B/NG)][GLCAP] the same for NG consecutive slots
 //   Veff                [n^3] periodic grid (x fastest), read through wrapped indices
 //   projector ELL       over the extended grid (n + 2 nloc)^3: per point the
 //                       (image-projector id, U value) pairs that cover it
@@ -21,6 +22,8 @@ namespace sqk {
 constexpr int FDN = 6;          // half FD order (12th-order stencil)
 constexpr int WIN = 2 * FDN + 1;
 constexpr int LCAP = 512;       // max images meeting one node cube
+constexpr int NG = 8;           // node cubes per workgroup of the grouped projector kernels
+constexpr int GLCAP = 1024;     // max images meeting the cubes of one group
 constexpr int MAXM = 43;        // largest cube side one workgroup marches ((M+1)/2*M <= 1024)
 constexpr int MAXIT = 500;      // Lanczos max_iter (sq.cpp:967)
 
@@ -441,6 +444,9 @@ struct PArgs {
     const int *nlocc, *nproj;     // per type
     const double *inv_denom;      // [type][16]
     const double *ca, *ib;        // LANCZOS: projects v = (V - ca V2) * ib
+    const int *glist, *glist_cnt; // group image lists (k_projg, k_projs)
+    int B;                        // slots in the batch (k_projg, k_projs)
+    int pu;                       // k_projs: largest union plane (points) staged at once
 };
 
 // s[b][ip] = <U_ip, v_b>/Denom over the intersection of the image's rc-cube
@@ -458,7 +464,7 @@ __global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
     const int node = G.node_list[b];
     const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
     const int cnt = A.list_cnt[b];
     const T *__restrict__ V = A.V + (size_t)b * G.M3;
     const T *__restrict__ V2 = FORMED ? A.V2 + (size_t)b * G.M3 : nullptr;
@@ -506,6 +512,482 @@ __global__ void __launch_bounds__(256) k_proj(PArgs<T> A)
     }
 }
 
+// The projections of k_proj for the NG consecutive slots of a group at once:
+// one wave per image of the group list reads each U value once for the
+// group (the cubes of neighbouring nodes overlap in all but a few planes)
+// and accumulates it against every member's vector.  Lane l always takes
+// the points of the image's rc-box whose xy index in the box is l mod 64,
+// z ascending, and a point outside a member's cube adds an exact zero
+// (buffer loads past the member's range read 0), so each member's sums do
+// not depend on which slots share its group.  Members are handled GS at a
+// time (GS accumulator sets of MAXP values per lane).
+template <typename T, int MAXP, int FORMED>
+__global__ void __launch_bounds__(256) k_projg(PArgs<T> A)
+{
+    constexpr int GS = MAXP <= 4 ? NG : MAXP <= 8 ? NG / 2 : NG / 4;
+    constexpr unsigned TS = sizeof(T);
+    const Geo &G = A.g;
+    const int q = blockIdx.x;
+    const int b0 = q * NG, gn = min(NG, A.B - b0);
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, M3 = G.M3;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
+    int ni[NG], nj[NG], nk[NG];
+    bool on[NG];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+        on[m] = m < gn && !(G.done && G.done[b0 + m]);
+        const int node = G.node_list[b0 + min(m, gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+        any |= on[m];
+    }
+    if (!any) return;
+    const size_t gb = (size_t)gn * M3 * TS;
+    const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
+    const __amdgpu_buffer_rsrc_t rY = brsrc((FORMED ? A.V2 : A.V) + (size_t)b0 * M3, gb);
+    const int cnt = A.glist_cnt[q];
+    for (int li = wave; li < cnt; li += nwave) {
+        const int J = A.glist[(size_t)q * GLCAP + li];
+        const int t = A.img_type[J];
+        const int nc = A.nlocc[t], np = A.nproj[t], w = 2 * nc + 1, w2 = w * w, w3 = w2 * w;
+        const int ox = A.img_nn[3 * J] - nc, oy = A.img_nn[3 * J + 1] - nc, oz = A.img_nn[3 * J + 2] - nc;
+        const __amdgpu_buffer_rsrc_t rU = brsrc(A.udv + A.udv_off[J], (size_t)np * w3 * TS);
+        const double *idn = A.inv_denom + 16 * t;
+        const int ipb = A.ip_base[J];
+#pragma unroll 1
+        for (int m0 = 0; m0 < NG; m0 += GS) {
+            // member ranges in box coordinates, and their union
+            int lx[GS], hx[GS], ly[GS], hy[GS], lz[GS], hz[GS];
+            int ulx = w, uhx = -1, uly = w, uhy = -1, ulz = w, uhz = -1;
+            bool ne[GS];
+#pragma unroll
+            for (int g = 0; g < GS; ++g) {
+                const int m = m0 + g;
+                lx[g] = max(0, ni[m] - nloc - ox); hx[g] = min(w - 1, ni[m] + nloc - ox);
+                ly[g] = max(0, nj[m] - nloc - oy); hy[g] = min(w - 1, nj[m] + nloc - oy);
+                lz[g] = max(0, nk[m] - nloc - oz); hz[g] = min(w - 1, nk[m] + nloc - oz);
+                ne[g] = on[m] && lx[g] <= hx[g] && ly[g] <= hy[g] && lz[g] <= hz[g];
+                if (!ne[g]) { lx[g] = w; hx[g] = -1; lz[g] = w; hz[g] = -1; }
+                else {
+                    ulx = min(ulx, lx[g]); uhx = max(uhx, hx[g]);
+                    uly = min(uly, ly[g]); uhy = max(uhy, hy[g]);
+                    ulz = min(ulz, lz[g]); uhz = max(uhz, hz[g]);
+                }
+            }
+            if (uhx < 0) continue;
+            T cab[GS], ibb[GS];
+#pragma unroll
+            for (int g = 0; g < GS; ++g) {
+                cab[g] = 0; ibb[g] = 1;
+                if (FORMED && ne[g]) { cab[g] = (T)A.ca[b0 + m0 + g]; ibb[g] = (T)A.ib[b0 + m0 + g]; }
+            }
+            double acc[GS][MAXP];
+#pragma unroll
+            for (int g = 0; g < GS; ++g)
+#pragma unroll
+                for (int i = 0; i < MAXP; ++i) acc[g][i] = 0.0;
+            const int a_hi = (uhy + 1) * w;
+            for (int a0 = (uly * w) & ~63; a0 < a_hi; a0 += 64) {
+                const int a = a0 + lane;
+                const int by = a / w, bx = a - by * w;
+                const bool inu = a < w2 && bx >= ulx && bx <= uhx && by >= uly && by <= uhy;
+                int base[GS];
+                bool inm[GS];
+#pragma unroll
+                for (int g = 0; g < GS; ++g) {
+                    const int m = m0 + g;
+                    inm[g] = inu && bx >= lx[g] && bx <= hx[g] && by >= ly[g] && by <= hy[g];
+                    base[g] = m * M3 + (oz - nk[m] + nloc) * M2 + (oy + by - nj[m] + nloc) * M +
+                              (ox + bx - ni[m] + nloc);
+                }
+#pragma unroll 1
+                for (int bz = ulz; bz <= uhz; ++bz) {
+                    T u[MAXP];
+#pragma unroll
+                    for (int i = 0; i < MAXP; ++i)
+                        u[i] = bload<T>(rU, (inu && i < np) ? (unsigned)(i * w3 + bz * w2 + a) * TS : OOB);
+#pragma unroll
+                    for (int g = 0; g < GS; ++g) {
+                        const bool ok = inm[g] && bz >= lz[g] && bz <= hz[g];
+                        const unsigned o = ok ? (unsigned)(base[g] + bz * M2) * TS : OOB;
+                        double v;
+                        if (FORMED) v = (double)((bload<T>(rX, o) - cab[g] * bload<T>(rY, o)) * ibb[g]);
+                        else v = (double)bload<T>(rX, o);
+#pragma unroll
+                        for (int i = 0; i < MAXP; ++i) acc[g][i] += v * (double)u[i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < GS; ++g) {
+                if (!ne[g]) continue;
+                double *sb = A.s + (size_t)(b0 + m0 + g) * A.n_ip + ipb;
+#pragma unroll
+                for (int i = 0; i < MAXP; ++i) {
+                    if (i < np) {
+                        const double r = wave_sum(acc[g][i]);
+                        if (lane == 0) sb[i] = r * idn[i];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Wave sums of NV values at once (NV a power of two <= 32): the values are
+// halved between lane pairs (xor 32, 16, ...) until each lane holds one, then
+// summed over the remaining lane bits.  Every value goes through the same
+// pairing tree as wave_sum (xor 32, 16, 8, 4, 2, 1), so each total is
+// bitwise the one wave_sum would give.  Returns, in lane l, the total of
+// value l >> (6 - log2 NV).
+template <int NV>
+__device__ __forceinline__ double wave_sums(double (&v)[NV], int lane)
+{
+#pragma unroll
+    for (int lv = 0; lv < 6; ++lv) {
+        const int o = 32 >> lv, h = (NV >> lv) / 2;   // h values move to the partner
+        if (h > 0) {
+            const bool up = (lane & o) != 0;
+#pragma unroll
+            for (int j = 0; j < NV / 2; ++j) {
+                if (j < h) {
+                    // the empty asm keeps the selects on register values (a
+                    // select between two array elements would otherwise become
+                    // a load from a selected address and move v to scratch)
+                    double lo = v[j], hi = v[j + h];
+                    asm volatile("" : "+v"(lo), "+v"(hi));
+                    const double send = up ? lo : hi;
+                    const double keep = up ? hi : lo;
+                    v[j] = keep + __shfl_xor(send, o, 64);
+                }
+            }
+        } else {
+            v[0] += __shfl_xor(v[0], o, 64);
+        }
+    }
+    return v[0];
+}
+
+// The projections of k_projg with every vector value read from HBM once per
+// group: the workgroup walks the union of the members' cubes plane by plane
+// (global z ascending), stages each plane of the members' (formed) vectors
+// in LDS as [point][member] with zeros outside a member's cube, and each
+// wave adds the plane's slice of its images (the same wave always takes the
+// same image) into per-image totals in LDS.  Lane l takes the box points
+// whose xy index in the image box is l mod 64; the plane's partial sums are
+// combined by wave_sums and added plane after plane, so a member's totals
+// depend only on its own cube.  A group whose union plane exceeds pu points
+// is walked one member at a time, in the same order.
+template <int MAXP> constexpr int projs_threads() { return 512; }
+
+// Image data that k_projs reads per image and plane, cached in LDS per group
+struct ImgMeta {
+    int ox, oy, oz;               // rc-box origin (unwrapped grid coordinates)
+    int w, np, t, ipb;            // box side, projectors, type, first image-projector id
+    long uoff;                    // offset of the image's U block in udv
+};
+
+template <typename T, int MAXP, int FORMED>
+__global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
+{
+    constexpr int NV = NG * MAXP;
+    constexpr int LV = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : 0;
+    static_assert(LV > 0, "k_projs: NG * MAXP must be 8, 16 or 32");
+    constexpr unsigned TS = sizeof(T);
+    constexpr int RPI = 3;   // rounds of 64 box points per U load batch (w <= 13 in one batch)
+    constexpr int PPT = 3;   // staged points per thread per load batch
+    const Geo &G = A.g;
+    const int q = blockIdx.x;
+    const int b0 = q * NG, gn = min(NG, A.B - b0);
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, M3 = G.M3;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwave = blockDim.x >> 6;
+    int ni[NG], nj[NG], nk[NG];
+    bool on[NG];
+    T cab[NG], ibb[NG];
+    int nact = 0;
+    unsigned onmask = 0;
+    int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+        on[m] = m < gn && !(G.done && G.done[b0 + m]);
+        const int node = G.node_list[b0 + min(m, gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+        cab[m] = 0; ibb[m] = 1;
+        if (on[m]) {
+            ++nact;
+            onmask |= 1u << m;
+            if (FORMED) { cab[m] = (T)A.ca[b0 + m]; ibb[m] = (T)A.ib[b0 + m]; }
+            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
+            y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
+        }
+    }
+    const int cnt = A.glist_cnt[q];
+    if (nact == 0 || cnt == 0) return;
+    // LDS: per-image totals [cnt][NV] | image data [cnt] | staged plane [pu][NG]
+    extern __shared__ unsigned char smem_raw[];
+    double *sacc = reinterpret_cast<double *>(smem_raw);
+    ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw + (size_t)cnt * NV * sizeof(double));
+    T *pl = reinterpret_cast<T *>(smem_raw + (size_t)cnt * NV * sizeof(double) + (size_t)cnt * sizeof(ImgMeta));
+    for (int i = tid; i < cnt * NV; i += blockDim.x) sacc[i] = 0.0;
+    for (int li = tid; li < cnt; li += blockDim.x) {
+        const int J = A.glist[(size_t)q * GLCAP + li];
+        const int t = A.img_type[J], nc = A.nlocc[t];
+        ImgMeta e;
+        e.ox = A.img_nn[3 * J] - nc; e.oy = A.img_nn[3 * J + 1] - nc; e.oz = A.img_nn[3 * J + 2] - nc;
+        e.w = 2 * nc + 1; e.np = A.nproj[t]; e.t = t; e.ipb = A.ip_base[J];
+        e.uoff = A.udv_off[J];
+        im[li] = e;
+    }
+    const size_t gb = (size_t)gn * M3 * TS;
+    const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
+    const __amdgpu_buffer_rsrc_t rY = brsrc((FORMED ? A.V2 : A.V) + (size_t)b0 * M3, gb);
+    const bool together = (x1 - x0 + M) * (y1 - y0 + M) <= A.pu;
+#pragma unroll 1
+    for (int sub = 0; sub < (together ? 1 : NG); ++sub) {
+        // members staged in this walk, and the union of their cubes
+        bool in_sub[NG];
+        int ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30), uz0 = 1 << 30, uz1 = -(1 << 30);
+#pragma unroll
+        for (int m = 0; m < NG; ++m) {
+            in_sub[m] = on[m] && (together || m == sub);
+            if (in_sub[m]) {
+                ux0 = min(ux0, ni[m] - nloc); ux1 = max(ux1, ni[m] + nloc);
+                uy0 = min(uy0, nj[m] - nloc); uy1 = max(uy1, nj[m] + nloc);
+                uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc);
+            }
+        }
+        if (ux1 < ux0) continue;
+        const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, up = ux * uy;
+#pragma unroll 1
+        for (int gz = uz0; gz <= uz1; ++gz) {
+            __syncthreads();   // the previous plane is consumed (and sacc zeroed)
+            // stage the plane: every load of the thread's points issued before
+            // the first store (one memory latency per plane)
+#pragma unroll 1
+            for (int p0 = 0; p0 < up; p0 += PPT * (int)blockDim.x) {
+                T xv[PPT][NG], yv[PPT][NG];
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    const int yy = p / ux, xx = p - yy * ux;
+                    const int gx = ux0 + xx, gy = uy0 + yy;
+#pragma unroll
+                    for (int m = 0; m < NG; ++m) {
+                        const int lx = gx - ni[m] + nloc, ly = gy - nj[m] + nloc, lz = gz - nk[m] + nloc;
+                        const bool ok = p < up && in_sub[m] && (unsigned)lx < (unsigned)M &&
+                                        (unsigned)ly < (unsigned)M && (unsigned)lz < (unsigned)M;
+                        const unsigned o = ok ? (unsigned)(m * M3 + lz * M2 + ly * M + lx) * TS : OOB;
+                        xv[r][m] = bload<T>(rX, o);
+                        yv[r][m] = FORMED ? bload<T>(rY, o) : (T)0;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    if (p < up) {
+#pragma unroll
+                        for (int m = 0; m < NG; ++m)
+                            pl[(size_t)p * NG + m] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int li = wave; li < cnt; li += nwave) {
+                // the image's data is wave-uniform: keep it in scalar registers
+                const ImgMeta &e = im[li];
+                const int w = __builtin_amdgcn_readfirstlane(e.w), oz = __builtin_amdgcn_readfirstlane(e.oz);
+                const int bz = gz - oz;
+                if ((unsigned)bz >= (unsigned)w) continue;
+                const int ox = __builtin_amdgcn_readfirstlane(e.ox), oy = __builtin_amdgcn_readfirstlane(e.oy);
+                const int rlo = max(0, uy0 - oy), rhi = min(w - 1, uy1 - oy);
+                const int clo = max(0, ux0 - ox), chi = min(w - 1, ux1 - ox);
+                if (rlo > rhi || clo > chi) continue;
+                const int np = __builtin_amdgcn_readfirstlane(e.np), w2 = w * w, w3 = w2 * w;
+                const long uoff = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(e.uoff >> 32)) << 32) |
+                                         (unsigned)__builtin_amdgcn_readfirstlane((int)e.uoff));
+                const __amdgpu_buffer_rsrc_t rU = brsrc(A.udv + uoff, (size_t)np * w3 * TS);
+                double acc[NV];
+#pragma unroll
+                for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+                const int a_hi = (rhi + 1) * w;
+                // RPI rounds of 64 box points at a time, their U loads issued together
+#pragma unroll 1
+                for (int c0 = (rlo * w) & ~63; c0 < a_hi; c0 += 64 * RPI) {
+                    T u[RPI][MAXP];
+                    int pp[RPI];
+                    bool in[RPI];
+#pragma unroll
+                    for (int r = 0; r < RPI; ++r) {
+                        const int a = c0 + 64 * r + lane;
+                        const int by = a / w, bx = a - by * w;
+                        in[r] = a < a_hi && by >= rlo && by <= rhi && bx >= clo && bx <= chi;
+#pragma unroll
+                        for (int i = 0; i < MAXP; ++i)
+                            u[r][i] = bload<T>(rU, (in[r] && i < np) ? (unsigned)(i * w3 + bz * w2 + a) * TS : OOB);
+                        pp[r] = in[r] ? (oy + by - uy0) * ux + (ox + bx - ux0) : 0;
+                    }
+#pragma unroll
+                    for (int r = 0; r < RPI; ++r) {
+                        const T *pv = pl + (size_t)pp[r] * NG;
+                        T v[NG];
+#pragma unroll
+                        for (int m = 0; m < NG; ++m) v[m] = pv[m];
+                        if (in[r]) {
+#pragma unroll
+                            for (int m = 0; m < NG; ++m)
+#pragma unroll
+                                for (int i = 0; i < MAXP; ++i)
+                                    acc[m * MAXP + i] += (double)v[m] * (double)u[r][i];
+                        }
+                    }
+                }
+                const double tot = wave_sums<NV>(acc, lane);
+                if ((lane & ((64 >> LV) - 1)) == 0) sacc[(size_t)li * NV + (lane >> (6 - LV))] += tot;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < cnt * NV; e += blockDim.x) {
+        const int li = e / NV, k = e - li * NV, m = k / MAXP, i = k - m * MAXP;
+        if (((onmask >> m) & 1u) && i < im[li].np)
+            A.s[(size_t)(b0 + m) * A.n_ip + im[li].ipb + i] = sacc[e] * A.inv_denom[16 * im[li].t + i];
+    }
+}
+
+// The nonlocal term of k_nlapply for the NG consecutive slots of a group:
+// each point of the union of the members' cubes reads its ELL entries once
+// and applies them to every member's projections, which sit in LDS
+// compacted to the images of the group list (loc maps an image-projector id
+// to its row, sl holds NG values per row).  Each member's sum runs over the
+// point's entries in slot order, as in k_nlapply.  When the members' cubes
+// are far apart (a group that spans two bricks), each member's cube is
+// walked on its own instead of the union box.
+template <typename T>
+struct NGArgs {
+    Geo g;
+    NL<T> nl;
+    T *nlv;                       // [B][M^3] out
+    const int *glist, *glist_cnt;
+    const int *img_type, *nproj, *ip_base;
+    int B;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(512) k_nlapplyg(NGArgs<T> A)
+{
+    constexpr unsigned TS = sizeof(T);
+    const Geo &G = A.g;
+    const int q = blockIdx.x;
+    const int b0 = q * NG, gn = min(NG, A.B - b0);
+    const int n = G.n, M = G.M, M2 = G.M2, M3 = G.M3, E = G.E;
+    int ni[NG], nj[NG], nk[NG];
+    bool on[NG];
+    int nact = 0;
+    int ex0 = 1 << 30, ex1 = -1, ey0 = 1 << 30, ey1 = -1, ez0 = 1 << 30, ez1 = -1;
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+        on[m] = m < gn && !(G.done && G.done[b0 + m]);
+        const int node = G.node_list[b0 + min(m, gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+        if (on[m]) {
+            ++nact;
+            ex0 = min(ex0, ni[m]); ex1 = max(ex1, ni[m] + M - 1);
+            ey0 = min(ey0, nj[m]); ey1 = max(ey1, nj[m] + M - 1);
+            ez0 = min(ez0, nk[m]); ez1 = max(ez1, nk[m] + M - 1);
+        }
+    }
+    if (nact == 0) return;
+    // LDS: loc [n_ip] | row offsets [cnt + 1] | sl [rows][NG]
+    extern __shared__ unsigned char smem_raw[];
+    int *loc = reinterpret_cast<int *>(smem_raw);
+    const int cnt = A.glist_cnt[q];
+    int *roff = loc + A.nl.n_ip;
+    double *sl = reinterpret_cast<double *>(smem_raw + (((size_t)A.nl.n_ip + cnt + 1) * 4 + 15) / 16 * 16);
+    const int *gl = A.glist + (size_t)q * GLCAP;
+    for (int i = threadIdx.x; i < A.nl.n_ip; i += blockDim.x) loc[i] = 0;
+    if (threadIdx.x == 0) {
+        int r = 0;
+        for (int l = 0; l < cnt; ++l) { roff[l] = r; r += A.nproj[A.img_type[gl[l]]]; }
+        roff[cnt] = r;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < cnt; l += blockDim.x) {
+        const int J = gl[l], r0 = roff[l], np = roff[l + 1] - r0, ipb = A.ip_base[J];
+        for (int i = 0; i < np; ++i) {
+            loc[ipb + i] = r0 + i;
+#pragma unroll
+            for (int m = 0; m < NG; ++m)
+                sl[(size_t)(r0 + i) * NG + m] = on[m] ? A.nl.s[(size_t)(b0 + m) * A.nl.n_ip + ipb + i] : 0.0;
+        }
+    }
+    if (cnt == 0 && threadIdx.x < NG) sl[threadIdx.x] = 0.0;
+    __syncthreads();
+    const int E3 = (int)A.nl.E3;
+    const int KA = A.nl.k_alloc;
+    const __amdgpu_buffer_rsrc_t rEU = brsrc(A.nl.u, (size_t)KA * E3 * TS);
+    const __amdgpu_buffer_rsrc_t rEI = brsrc(A.nl.ip, (size_t)KA * E3 * 4);
+    const __amdgpu_buffer_rsrc_t rEC = brsrc(A.nl.cnt, (size_t)E3);
+    T *out = A.nlv + (size_t)b0 * M3;
+    // one extended-grid point: sum its entries against every member's
+    // projections, four slots at a time up to the wave's largest count (a
+    // slot past the point's count loads u = 0 and adds exact zeros)
+    constexpr int CH = 4;
+    auto point = [&](int gx, int gy, int gz, double a[NG]) {
+        const int g = (gz * E + gy) * E + gx;
+        const int cn = bload_u8(rEC, (unsigned)g);
+#pragma unroll
+        for (int m = 0; m < NG; ++m) a[m] = 0.0;
+#pragma unroll 1
+        for (int j0 = 0; j0 < KA && __ballot(cn > j0) != 0; j0 += CH) {
+            T u[CH];
+            int ip[CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const int j = j0 + c;
+                const bool ok = j < cn;
+                u[c] = bload<T>(rEU, ok ? (unsigned)(j * E3 + g) * TS : OOB);
+                ip[c] = bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * 4u : OOB);
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const double *row = sl + (size_t)loc[ip[c]] * NG;
+#pragma unroll
+                for (int m = 0; m < NG; ++m) a[m] += row[m] * (double)u[c];
+            }
+        }
+    };
+    const int ux = ex1 - ex0 + 1, uy = ey1 - ey0 + 1, uz = ez1 - ez0 + 1;
+    if ((long)ux * uy * uz <= (long)nact * M3) {
+        const int uxy = ux * uy, tot = uxy * uz;
+#pragma unroll 1
+        for (int p = threadIdx.x; p < tot; p += blockDim.x) {
+            const int z = p / uxy, r = p - z * uxy, y = r / ux, x = r - y * ux;
+            const int gx = ex0 + x, gy = ey0 + y, gz = ez0 + z;
+            double a[NG];
+            point(gx, gy, gz, a);
+#pragma unroll
+            for (int m = 0; m < NG; ++m) {
+                const unsigned lx = gx - ni[m], ly = gy - nj[m], lz = gz - nk[m];
+                if (on[m] && lx < (unsigned)M && ly < (unsigned)M && lz < (unsigned)M)
+                    out[(size_t)m * M3 + (lz * M2 + ly * M + lx)] = (T)a[m];
+            }
+        }
+    } else {
+        for (int m = 0; m < NG; ++m) {
+            if (!on[m]) continue;
+#pragma unroll 1
+            for (int p = threadIdx.x; p < M3; p += blockDim.x) {
+                const int z = p / M2, r = p - z * M2, y = r / M, x = r - y * M;
+                double a[NG];
+                point(ni[m] + x, nj[m] + y, nk[m] + z, a);
+                out[(size_t)m * M3 + p] = (T)a[m];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- nonlocal forces
 constexpr int FSLOT = 32;   // images whose rc-box holds one node centre
 
@@ -537,7 +1019,7 @@ __global__ void __launch_bounds__(256) k_force_contract(FArgs<T> A)
     const int node = G.node_list[b];
     const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwave = blockDim.x >> 6;
     const int cnt = A.list_cnt[b];
     __shared__ int slot_of[LCAP];
     if (threadIdx.x == 0) {
@@ -623,6 +1105,7 @@ struct LArgs {
     int n_img;
     const int *img_type, *img_nn, *nlocc, *nproj;
     int *list, *list_cnt, *overflow;
+    int *glist, *glist_cnt;       // [ceil(B/NG)][GLCAP], [ceil(B/NG)]
 };
 
 __global__ void k_node_images(LArgs A, int B)
@@ -653,6 +1136,45 @@ __global__ void k_node_images(LArgs A, int B)
         L[j + 1] = J;
     }
     A.list_cnt[b] = cnt;
+}
+
+// Images meeting any of the cubes of the NG consecutive slots of a group
+// (the union of their node lists), sorted by z like the node lists.
+__global__ void k_group_images(LArgs A, int B)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (B + NG - 1) / NG) return;
+    const int b0 = q * NG, gn = min(NG, B - b0);
+    const int n = A.g.n, nloc = A.g.nloc;
+    int ni[NG], nj[NG], nk[NG];
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+        const int node = A.g.node_list[b0 + min(m, gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+    }
+    int cnt = 0;
+    int *L = A.glist + (size_t)q * GLCAP;
+    for (int J = 0; J < A.n_img; ++J) {
+        const int t = A.img_type[J];
+        if (A.nproj[t] == 0) continue;
+        const int R = nloc + A.nlocc[t];
+        const int cx = A.img_nn[3 * J], cy = A.img_nn[3 * J + 1], cz = A.img_nn[3 * J + 2];
+        bool hit = false;
+#pragma unroll
+        for (int m = 0; m < NG; ++m)
+            hit |= abs(cx - ni[m]) <= R && abs(cy - nj[m]) <= R && abs(cz - nk[m]) <= R;
+        if (!hit) continue;
+        if (cnt < GLCAP) L[cnt] = J;
+        ++cnt;
+    }
+    if (cnt > GLCAP) { atomicOr(A.overflow, 1); cnt = GLCAP; }
+    for (int i = 1; i < cnt; ++i) {
+        const int J = L[i], kz = A.img_nn[3 * J + 2];
+        int j = i - 1;
+        while (j >= 0 && A.img_nn[3 * L[j] + 2] > kz) { L[j + 1] = L[j]; --j; }
+        L[j + 1] = J;
+    }
+    A.glist_cnt[q] = cnt;
 }
 
 // ---------------------------------------------------------------- fills

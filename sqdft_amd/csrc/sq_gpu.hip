@@ -457,8 +457,10 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     if (std::getenv("SQDFT_DEBUG_LANCZOS")) {
         // diagnostic: alpha/beta of the first node of the batch
         std::vector<double> ha(MAXIT + 2), hb(MAXIT + 2);
-        HIPCHK(hipMemcpy(ha.data(), c->aa.p, ha.size() * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(hb.data(), c->bb.p, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy2D(ha.data(), sizeof(double), c->aa.p, (size_t)B * sizeof(double), sizeof(double),
+                           ha.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy2D(hb.data(), sizeof(double), c->bb.p, (size_t)B * sizeof(double), sizeof(double),
+                           hb.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < 120; ++i) std::fprintf(stderr, "lanczos b0 pass %d a=%.17g b=%.17g\n", i, ha[i], hb[i]);
     }
     return SQG_OK;

@@ -640,13 +640,41 @@ __global__ void __launch_bounds__(256) k_projg(PArgs<T> A)
 // pairing tree as wave_sum (xor 32, 16, 8, 4, 2, 1), so each total is
 // bitwise the one wave_sum would give.  Returns, in lane l, the total of
 // value l >> (6 - log2 NV).
+// One halving step between lanes l and l ^ 32 (O = 32) or l ^ 16 (O = 16)
+// with the gfx950 half-wave swaps: permlane{32,16}_swap(p, q) leaves the
+// lanes whose bit O is clear with (own p, partner's p) and the others with
+// (partner's q, own q), so their sum is p or q summed over the lane pair.
+template <int O>
+__device__ __forceinline__ double swap_sum(double p, double q)
+{
+    const unsigned long long pb = __builtin_bit_cast(unsigned long long, p);
+    const unsigned long long qb = __builtin_bit_cast(unsigned long long, q);
+    unsigned lo0, lo1, hi0, hi1;
+    if (O == 32) {
+        const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)pb, (unsigned)qb, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(pb >> 32), (unsigned)(qb >> 32), false, false);
+        lo0 = lo[0]; lo1 = lo[1]; hi0 = hi[0]; hi1 = hi[1];
+    } else {
+        const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)pb, (unsigned)qb, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(pb >> 32), (unsigned)(qb >> 32), false, false);
+        lo0 = lo[0]; lo1 = lo[1]; hi0 = hi[0]; hi1 = hi[1];
+    }
+    const double a = __builtin_bit_cast(double, ((unsigned long long)hi0 << 32) | lo0);
+    const double b = __builtin_bit_cast(double, ((unsigned long long)hi1 << 32) | lo1);
+    return a + b;
+}
+
 template <int NV>
 __device__ __forceinline__ double wave_sums(double (&v)[NV], int lane)
 {
 #pragma unroll
     for (int lv = 0; lv < 6; ++lv) {
         const int o = 32 >> lv, h = (NV >> lv) / 2;   // h values move to the partner
-        if (h > 0) {
+        if (h > 0 && o >= 16) {
+#pragma unroll
+            for (int j = 0; j < NV / 2; ++j)
+                if (j < h) v[j] = o == 32 ? swap_sum<32>(v[j], v[j + h]) : swap_sum<16>(v[j], v[j + h]);
+        } else if (h > 0) {
             const bool up = (lane & o) != 0;
 #pragma unroll
             for (int j = 0; j < NV / 2; ++j) {
@@ -671,9 +699,11 @@ __device__ __forceinline__ double wave_sums(double (&v)[NV], int lane)
 // The projections of k_projg with every vector value read from HBM once per
 // group: the workgroup walks the union of the members' cubes plane by plane
 // (global z ascending), stages each plane of the members' (formed) vectors
-// in LDS as [point][member] with zeros outside a member's cube, and each
-// wave adds the plane's slice of its images (the same wave always takes the
-// same image) into per-image totals in LDS.  Lane l takes the box points
+// in LDS as [point][member] with zeros outside a member's cube, and the
+// images crossing the plane (one range of the z-sorted group list) are dealt
+// to the waves: one wave sums an image's slice of the plane and adds it to
+// the image's totals in LDS (planes are separated by barriers, so the totals
+// grow in plane order whichever wave did the sum).  Lane l takes the box points
 // whose xy index in the image box is l mod 64; the plane's partial sums are
 // combined by wave_sums and added plane after plane, so a member's totals
 // depend only on its own cube.  A group whose union plane exceeds pu points
@@ -686,6 +716,19 @@ struct ImgMeta {
     int w, np, t, ipb;            // box side, projectors, type, first image-projector id
     long uoff;                    // offset of the image's U block in udv
 };
+
+// First index of the z-sorted group list whose image centre z exceeds zc - 1
+// (centre z = oz + (w - 1) / 2).
+__device__ __forceinline__ int first_cz_at_least(const ImgMeta *im, int cnt, int zc)
+{
+    int lo = 0, hi = cnt;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (im[mid].oz + (im[mid].w - 1) / 2 < zc) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
 
 template <typename T, int MAXP, int FORMED>
 __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
@@ -725,6 +768,9 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
     const int cnt = A.glist_cnt[q];
     if (nact == 0 || cnt == 0) return;
     // LDS: per-image totals [cnt][NV] | image data [cnt] | staged plane [pu][NG]
+    __shared__ int sh_ncmax, sh_lo, sh_hi;
+    if (tid == 0) sh_ncmax = 0;
+    __syncthreads();
     extern __shared__ unsigned char smem_raw[];
     double *sacc = reinterpret_cast<double *>(smem_raw);
     ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw + (size_t)cnt * NV * sizeof(double));
@@ -738,6 +784,7 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
         e.w = 2 * nc + 1; e.np = A.nproj[t]; e.t = t; e.ipb = A.ip_base[J];
         e.uoff = A.udv_off[J];
         im[li] = e;
+        atomicMax(&sh_ncmax, nc);
     }
     const size_t gb = (size_t)gn * M3 * TS;
     const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
@@ -762,6 +809,11 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
 #pragma unroll 1
         for (int gz = uz0; gz <= uz1; ++gz) {
             __syncthreads();   // the previous plane is consumed (and sacc zeroed)
+            // the images that can cross plane gz form one range of the z-sorted list
+            if (tid == 0) {
+                sh_lo = first_cz_at_least(im, cnt, gz - sh_ncmax);
+                sh_hi = first_cz_at_least(im, cnt, gz + sh_ncmax + 1);
+            }
             // stage the plane: every load of the thread's points issued before
             // the first store (one memory latency per plane)
 #pragma unroll 1
@@ -793,8 +845,9 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
                 }
             }
             __syncthreads();
+            const int li_lo = __builtin_amdgcn_readfirstlane(sh_lo), li_hi = __builtin_amdgcn_readfirstlane(sh_hi);
 #pragma unroll 1
-            for (int li = wave; li < cnt; li += nwave) {
+            for (int li = li_lo + wave; li < li_hi; li += nwave) {
                 // the image's data is wave-uniform: keep it in scalar registers
                 const ImgMeta &e = im[li];
                 const int w = __builtin_amdgcn_readfirstlane(e.w), oz = __builtin_amdgcn_readfirstlane(e.oz);
@@ -1201,7 +1254,7 @@ struct QArgs {
     int B, pass;
     double tol;
     const double *lred;       // [B][4]
-    double *aa, *bb;          // [B][MAXIT+2]
+    double *aa, *bb;          // [MAXIT+2][B] (pass-major: the lanes of a wave read one row)
     double *d, *e;            // scratch [B][MAXIT+2]
     double *lmin, *lmax, *lmin_prev, *lmax_prev;
     int *count, *done;
@@ -1268,7 +1321,9 @@ __device__ int ql_extremes(double *d, double *e, int n, double *lmin, double *lm
 // characteristic polynomial is monotone and convex (concave on the left), so
 // the iterates move monotonically onto the root and the loop stops at the
 // first step that no longer moves inward.  Returns false after maxit steps.
-__device__ bool tridiag_extreme(const double *d, const double *e, int n, double x0, int sign, double *root)
+// d and e are strided by st elements.
+__device__ bool tridiag_extreme(const double *d, const double *e, size_t st, int n, double x0, int sign,
+                                double *root)
 {
     double x = x0;
     for (int it = 0; it < 200; ++it) {
@@ -1276,9 +1331,10 @@ __device__ bool tridiag_extreme(const double *d, const double *e, int n, double 
         double iq = 1.0 / q;
         double s = dq * iq;
         for (int i = 1; i < n; ++i) {
-            const double r = e[i - 1] * e[i - 1] * iq;
+            const double em = e[(size_t)(i - 1) * st];
+            const double r = em * em * iq;
             dq = 1.0 + r * dq * iq;
-            q = (x - d[i]) - r;
+            q = (x - d[(size_t)i * st]) - r;
             iq = 1.0 / q;
             s += dq * iq;
         }
@@ -1304,8 +1360,9 @@ __global__ void k_lanczos_scalars(QArgs A)
     double b2 = r[1] - 2.0 * al * r[2] + al * al * r[3];
     if (b2 < 0.0) b2 = 0.0;
     const double be = sqrt(b2);
-    A.aa[(size_t)b * S + p] = al;
-    A.bb[(size_t)b * S + p] = be;
+    const size_t st = (size_t)A.B;
+    A.aa[(size_t)p * st + b] = al;
+    A.bb[(size_t)p * st + b] = be;
     A.ca[b] = al;
     A.ib[b] = 1.0 / be;
     A.bprev[b] = be;
@@ -1316,22 +1373,22 @@ __global__ void k_lanczos_scalars(QArgs A)
     // extremes are used, and adding row p moves them by at most |beta_{p-1}|
     // beyond max(lambda_max(T_p), alpha_p) (min for lambda_min), which gives
     // Newton a start outside the spectrum.  QL stays as the fallback.
-    const double *ad = A.aa + (size_t)b * S, *be2 = A.bb + (size_t)b * S;
+    const double *ad = A.aa + b, *be2 = A.bb + b;   // row i at [i * st]
     double lo, hi, up, dn;
     if (p == 1) {   // Gershgorin
         const double r = fabs(be2[0]);
-        up = fmax(ad[0], ad[1]) + r;
-        dn = fmin(ad[0], ad[1]) - r;
+        up = fmax(ad[0], ad[st]) + r;
+        dn = fmin(ad[0], ad[st]) - r;
     } else {
-        const double r = fabs(be2[p - 1]);
-        up = fmax(A.lmax_prev[b], ad[p]) + r;
-        dn = fmin(A.lmin_prev[b], ad[p]) - r;
+        const double r = fabs(be2[(size_t)(p - 1) * st]);
+        up = fmax(A.lmax_prev[b], ad[(size_t)p * st]) + r;
+        dn = fmin(A.lmin_prev[b], ad[(size_t)p * st]) - r;
     }
     up += 8.0 * 2.2e-16 * fabs(up) + 1e-300;
     dn -= 8.0 * 2.2e-16 * fabs(dn) + 1e-300;
-    if (!tridiag_extreme(ad, be2, nq, up, 1, &hi) || !tridiag_extreme(ad, be2, nq, dn, -1, &lo)) {
+    if (!tridiag_extreme(ad, be2, st, nq, up, 1, &hi) || !tridiag_extreme(ad, be2, st, nq, dn, -1, &lo)) {
         double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
-        for (int i = 0; i < nq; ++i) { d[i] = ad[i]; e[i] = be2[i]; }
+        for (int i = 0; i < nq; ++i) { d[i] = ad[(size_t)i * st]; e[i] = be2[(size_t)i * st]; }
         if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
     }
     const double dl = fabs(lo - A.lmin_prev[b]), dm = fabs(hi - A.lmax_prev[b]);

@@ -147,17 +147,20 @@ def main():
     value = nn / (ms * 1e-3)
     st = last["stats"]
     # dominant z-march kernel: HIP events around every launch of the last step
-    kt = [sq.kernel_timing(0), sq.kernel_timing(1)]
-    keys = ["k_zmarch<CHEB>", "k_zmarch<LANCZOS>"]
+    # hot kernels, timed with HIP events around every launch of the last step
+    kt = [sq.kernel_timing(w) for w in range(4)]
+    keys = ["k_zmarch<CHEB>", "k_zmarch<LANCZOS>", "k_projs<LANCZOS>", "k_projs<CHEB>"]
     what = ["Chebyshev recurrence: read t_k, t_{k-1}, write t_{k+1}",
-            "Lanczos pass: read r', v_{k-1}, write v_k, r'_{k+1}"]
-    nvec = [3, 4]
+            "Lanczos pass: read r', v_{k-1}, write v_k, r'_{k+1}",
+            "projections of the Lanczos vector: read r', v_{k-1}",
+            "projections of t_k: read t_k"]
+    nvec = [3, 4, 2, 1]
     dom = int(np.argmax([k["total_ms"] for k in kt]))
     rs = 8 if args.precision == "fp64" else 4
     N = (2 * info["nloc"] + 1) ** 3
     traffic = profiled_traffic(args.workload, args.precision)
     rl = {}
-    for w in (0, 1):
+    for w in range(4):
         k = kt[w]
         avg_ms = k["total_ms"] / max(k["launches"], 1)
         alg = k["alg_bytes"] / max(k["launches"], 1)
@@ -170,8 +173,8 @@ def main():
             "traffic": rl[dom]["traffic"],
             "alg_bytes_per_unit": rl[dom]["alg_bytes_per_unit"],
             "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a plain copy "
-                              "(profiles/zmarch_traffic_lih216.md)" if rl[dom]["traffic"] else None,
-            "other": rl[1 - dom]}
+                              "(profiles/traffic_lih216.md)" if rl[dom]["traffic"] else None,
+            "others": [rl[w] for w in range(4) if w != dom]}
     out = {"metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",

@@ -127,9 +127,13 @@ int sqg_node_moments(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
 int sqg_hsub_apply_batch(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
                          const double *v, double *hv);
 
-/* Device kernel time (ms) of the dominant recurrence kernel over the last
- * sqg_density call: launches, summed and mean duration, and algorithmic
- * bytes per launch (3 * nodes * (2nloc+1)^3 * sizeof(real)). */
+/* Device time (ms, HIP events) of the hot kernels over the last density
+ * call: launches, summed duration and summed algorithmic bytes (N =
+ * (2nloc+1)^3 values per node, s = sizeof(real)).  which: 0 Chebyshev
+ * z-march (3 N s per node: t_k, t_{k-1} read, t_{k+1} written), 1 Lanczos
+ * z-march (4 N s: r', v_{k-1} read, v_k, r'_{k+1} written), 2 projections of
+ * the Lanczos passes (2 N s: the two stored vectors read), 3 projections of
+ * the Chebyshev steps (N s). */
 int sqg_last_kernel_timing(sqg_ctx *ctx, int which, long *launches, double *total_ms,
                            double *alg_bytes);
 

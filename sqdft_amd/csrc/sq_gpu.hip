@@ -85,13 +85,16 @@ struct sqg_ctx {
     DBuf adj, adj_sgn, adj_dl, adj_wt, adj_in, adj_out, adj_rec;
     bool have_state = false;
     double lambda_f = 0.0;
-    // timing of the dominant kernels (which: 0 Chebyshev z-march, 1 Lanczos
-    // z-march) and of the whole Lanczos phase of each batch (2)
-    std::vector<EventPair> ev[3];
+    // device timing of the hot kernels over the last density call (which:
+    // 0 Chebyshev z-march, 1 Lanczos z-march, 2 projections of the Lanczos
+    // passes, 3 projections of the Chebyshev steps) and of the whole Lanczos
+    // phase of each batch (EV_LANCZOS)
+    static constexpr int NEV = 5, EV_LANCZOS = 4;
+    std::vector<EventPair> ev[NEV];
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
-    long last_launches[3] = {0, 0, 0};
-    double last_ms[3] = {0, 0, 0}, last_bytes[3] = {0, 0, 0};
+    long last_launches[NEV] = {};
+    double last_ms[NEV] = {}, last_bytes[NEV] = {};
 
     size_t real_size() const { return p.precision == SQG_MIXED ? sizeof(float) : sizeof(double); }
     ~sqg_ctx()
@@ -302,8 +305,14 @@ void projs_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
 // Projections s = <U, v>/Denom of the B vectors (k_proj), then their
 // nonlocal term NL = sum U s into NLv (k_nlapply) for the next z-march.
 template <typename T, int FORMED>
-void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
+void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A, int which = -1, double alg_bytes = 0.0)
 {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (which >= 0 && c->n_ip > 0) {
+        e0 = c->ev_get();
+        e1 = c->ev_get();
+        if (e0) (void)hipEventRecord(e0, c->st);
+    }
     if (c->n_ip > 0) {
         PArgs<T> P = A;
         P.B = B;
@@ -332,6 +341,10 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A)
             hipLaunchKernelGGL((k_projg<T, 16, FORMED>), grid, dim3(256), 0, c->st, P);
     }
 apply:
+    if (e0 && e1) {
+        (void)hipEventRecord(e1, c->st);
+        c->ev[which].push_back({e0, e1, alg_bytes});
+    }
     NArgs<T> N{};
     N.g = A.g;
     N.nl.cnt = c->ell_cnt.as<unsigned char>();
@@ -432,7 +445,7 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     for (int pass = 0; pass <= MAXIT && active > 0; ++pass) {
         PArgs<T> P = pargs<T>(c, dn, ddone, X);
         P.V2 = Y;
-        launch_proj<T, 1>(c, B, P);
+        launch_proj<T, 1>(c, B, P, 2, 2.0 * active * c->M3 * rs);   // reads r' and v_{k-1}
         ZArgs<T> Z = zargs<T>(c, dn, ddone);
         Z.X = X; Z.Y = Y;
         Z.ca = c->ca.as<double>(); Z.ib = c->ib.as<double>(); Z.bprev = c->bprev.as<double>();
@@ -452,7 +465,7 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     if (passes_out) *passes_out += passes;
     if (ph0 && ph1) {
         (void)hipEventRecord(ph1, c->st);
-        c->ev[2].push_back({ph0, ph1, 0.0});
+        c->ev[sqg_ctx::EV_LANCZOS].push_back({ph0, ph1, 0.0});
     }
     if (std::getenv("SQDFT_DEBUG_LANCZOS")) {
         // diagnostic: alpha/beta of the first node of the batch
@@ -502,7 +515,7 @@ int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
     for (int k = 1; k <= c->p.npl / 2; ++k) {
         T *in = bufs[(k - 1) & 1], *out = bufs[k & 1];
         PArgs<T> P = pargs<T>(c, dn, nullptr, in);
-        launch_proj<T, 0>(c, B, P);
+        launch_proj<T, 0>(c, B, P, 3, (double)B * c->M3 * rs);   // reads t_k
         ZArgs<T> Z = zargs<T>(c, dn, nullptr);
         Z.X = in; Z.Y = out;
         Z.chi = c->chiB.as<double>(); Z.zee = c->zeeB.as<double>();
@@ -790,6 +803,19 @@ int rho_assembly(sqg_ctx *c, double lf, double *rho, double *Ebs, double *Eent, 
 
 #include "sq_adjoint_host.hpp"
 
+// Event bookkeeping of one call: reset before its first launch, summed after
+// its stream has synchronised.
+void timing_reset(sqg_ctx *c)
+{
+    for (int w = 0; w < sqg_ctx::NEV; ++w) c->ev[w].clear();
+    c->pool_used = 0;
+}
+double event_sum_ms(sqg_ctx *c, int which, double *bytes, long *launches);
+void timing_sum(sqg_ctx *c)
+{
+    for (int w = 0; w < sqg_ctx::NEV; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+}
+
 double event_sum_ms(sqg_ctx *c, int which, double *bytes, long *launches)
 {
     double ms = 0.0, by = 0.0;
@@ -983,12 +1009,11 @@ int sqg_node_moments(sqg_ctx *c, const double *veff, const int *nodes, int nb, d
     if (!st) st = ensure_node_state(c);
     if (st) return st;
     c->have_state = false;
-    for (int w = 0; w < 3; ++w) c->ev[w].clear();
-    c->pool_used = 0;
+    timing_reset(c);
     long passes = 0;
     st = moments_dispatch(c, nodes, nb, &passes);
     if (st) return st;
-    for (int w = 0; w < 3; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+    timing_sum(c);
     const size_t K = (size_t)c->p.npl + 1;
     std::vector<double> a(c->nn), z(c->nn);
     std::vector<int> it(c->nn);
@@ -1020,8 +1045,7 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     if (!st) st = ensure_node_state(c);
     if (st) return st;
     c->have_state = false;
-    for (int w = 0; w < 3; ++w) c->ev[w].clear();
-    c->pool_used = 0;
+    timing_reset(c);
     HIPCHK(hipEventRecord(t0, c->st));
     const int cap = choose_batch(c, c->nn);
     const std::vector<int> all = node_order(c->p.n);
@@ -1067,14 +1091,14 @@ int sqg_density(sqg_ctx *c, const double *veff, int scf_iter, double ncharge, do
     if (ebs) *ebs = Ebs;
     if (eent) *eent = Eent;
     c->have_state = true;
-    for (int w = 0; w < 3; ++w) c->last_ms[w] = event_sum_ms(c, w, &c->last_bytes[w], &c->last_launches[w]);
+    timing_sum(c);
     if (stats) {
         float a = 0, b = 0, d = 0;
         (void)hipEventElapsedTime(&a, t0, t1);
         (void)hipEventElapsedTime(&b, t1, t2);
         (void)hipEventElapsedTime(&d, t2, t3);
         std::memset(stats, 0, sizeof(*stats));
-        stats->t_lanczos = c->last_ms[2] * 1e-3;
+        stats->t_lanczos = c->last_ms[sqg_ctx::EV_LANCZOS] * 1e-3;
         stats->t_cheb = a * 1e-3 - stats->t_lanczos;
         stats->t_fermi = b * 1e-3;
         stats->t_rho = d * 1e-3;
@@ -1210,7 +1234,7 @@ int sqg_forces_nonlocal(sqg_ctx *c, double *f)
 
 int sqg_last_kernel_timing(sqg_ctx *c, int which, long *launches, double *total_ms, double *alg_bytes)
 {
-    if (!c || which < 0 || which > 1) return SQG_E_ARG;
+    if (!c || which < 0 || which > 3) return SQG_E_ARG;
     if (launches) *launches = c->last_launches[which];
     if (total_ms) *total_ms = c->last_ms[which];
     if (alg_bytes) *alg_bytes = c->last_bytes[which];
@@ -1229,8 +1253,11 @@ int sqg_density_fixed(sqg_ctx *c, const double *veff, double ncharge, double lf_
     const int cap = choose_batch(c, c->nn);
     const std::vector<int> all = node_order(c->p.n);
     long passes = 0;
+    timing_reset(c);
     st = moments_dispatch(c, all.data(), c->nn, &passes, cap, false);
     if (st) return st;
+    HIPCHK(hipStreamSynchronize(c->st));
+    timing_sum(c);
     const size_t N = (size_t)c->nn, K = (size_t)c->p.npl + 1;
     HIPCHK(c->W.ensure(N * K * sizeof(double)));
     hipLaunchKernelGGL(k_cheb_W, dim3(grid1(N * K, 256)), dim3(256), 0, c->st, c->mu_all.as<double>(),

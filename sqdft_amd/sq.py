@@ -177,7 +177,9 @@ class SpectralQuadrature:
         return out
 
     def kernel_timing(self, which: int = 0) -> dict:
-        """which: 0 Chebyshev recurrence z-march, 1 Lanczos z-march (last density call)."""
+        """Device time of the last density call's hot kernels.  which: 0 Chebyshev
+        z-march, 1 Lanczos z-march, 2 projections of the Lanczos passes, 3
+        projections of the Chebyshev steps."""
         n = C.c_long(); ms = C.c_double(); by = C.c_double()
         L.check("sqg_last_kernel_timing", self._lib.sqg_last_kernel_timing(
             self._ctx, which, C.byref(n), C.byref(ms), C.byref(by)))

@@ -294,12 +294,39 @@ void nlapply_go(sqg_ctx *c, int B, const NArgs<T> &A)
     hipLaunchKernelGGL(fn, dim3(B), dim3(512), shm, c->st, A);
 }
 
-template <typename T, int MAXP, int FORMED>
+template <typename T, int MAXP, int FORMED, int SZ, int NT>
 void projs_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
 {
-    auto fn = k_projs<T, MAXP, FORMED>;
+    auto fn = k_projs<T, MAXP, FORMED, SZ, NT>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(fn, grid, dim3(projs_threads<MAXP>()), shm, c->st, P);
+    hipLaunchKernelGGL(fn, grid, dim3(NT), shm, c->st, P);
+}
+
+// Staged projections with the deepest slab whose planes fit in LDS; false
+// when not even a single cube plane fits.
+template <typename T, int FORMED>
+bool launch_projs(sqg_ctx *c, dim3 grid, PArgs<T> &P, int mp)
+{
+    constexpr int D = sizeof(T) == 4 ? 4 : 2;   // deep slab (planes)
+    constexpr int H = D / 2;
+    constexpr int NT = 512;
+    const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
+    for (int pu : {(c->M + 3) * (c->M + 3), c->M2}) {
+        for (int sz : {D, H}) {
+            const size_t shs = sa + (size_t)pu * NG * sizeof(T) * sz;
+            if (shs > 160 * 1024) continue;
+            P.pu = pu;
+            if (sz == D) {
+                if (mp == 1) projs_go<T, 1, FORMED, D, NT>(c, grid, shs, P);
+                else projs_go<T, 4, FORMED, D, NT>(c, grid, shs, P);
+            } else {
+                if (mp == 1) projs_go<T, 1, FORMED, H, NT>(c, grid, shs, P);
+                else projs_go<T, 4, FORMED, H, NT>(c, grid, shs, P);
+            }
+            return true;
+        }
+    }
+    return false;
 }
 
 // Projections s = <U, v>/Denom of the B vectors (k_proj), then their
@@ -317,20 +344,10 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A, int which = -1, double al
         PArgs<T> P = A;
         P.B = B;
         const dim3 grid((B + NG - 1) / NG);
-        // staged projections (k_projs) when the per-image totals and a union
-        // plane of a 4x4 footprint (else a single cube plane) fit in LDS
+        // staged projections (k_projs) with the deepest slab whose planes
+        // (a union of a 4x4 footprint, else a single cube plane) fit in LDS
         const int mp = c->max_nproj <= 1 ? 1 : c->max_nproj <= 4 ? 4 : 0;
-        if (mp) {
-            const size_t sa = (size_t)std::max(c->gl_max, 1) * (NG * mp * sizeof(double) + sizeof(ImgMeta));
-            for (int pu : {(c->M + 3) * (c->M + 3), c->M2}) {
-                const size_t shs = sa + (size_t)pu * NG * sizeof(T);
-                if (shs > 160 * 1024) continue;
-                P.pu = pu;
-                if (mp == 1) projs_go<T, 1, FORMED>(c, grid, shs, P);
-                else projs_go<T, 4, FORMED>(c, grid, shs, P);
-                goto apply;
-            }
-        }
+        if (mp && launch_projs<T, FORMED>(c, grid, P, mp)) goto apply;
         if (c->max_nproj <= 1)
             hipLaunchKernelGGL((k_projg<T, 1, FORMED>), grid, dim3(256), 0, c->st, P);
         else if (c->max_nproj <= 4)

@@ -701,14 +701,15 @@ __device__ __forceinline__ double wave_sums(double (&v)[NV], int lane)
 // (global z ascending), stages each plane of the members' (formed) vectors
 // in LDS as [point][member] with zeros outside a member's cube, and the
 // images crossing the plane (one range of the z-sorted group list) are dealt
-// to the waves: one wave sums an image's slice of the plane and adds it to
-// the image's totals in LDS (planes are separated by barriers, so the totals
-// grow in plane order whichever wave did the sum).  Lane l takes the box points
+// to the waves: one wave sums an image's slice of the slab and adds it,
+// divided by Denom, to the image's projections in memory with one hardware
+// FP64 atomic per value (slabs are separated by a fence and a barrier, so
+// each projection grows slab by slab in z order whichever wave did the sum).
+// Slabs hold SZ planes whose first global z is a multiple of SZ.  Lane l takes the box points
 // whose xy index in the image box is l mod 64; the plane's partial sums are
 // combined by wave_sums and added plane after plane, so a member's totals
 // depend only on its own cube.  A group whose union plane exceeds pu points
 // is walked one member at a time, in the same order.
-template <int MAXP> constexpr int projs_threads() { return 512; }
 
 // Image data that k_projs reads per image and plane, cached in LDS per group
 struct ImgMeta {
@@ -730,15 +731,15 @@ __device__ __forceinline__ int first_cz_at_least(const ImgMeta *im, int cnt, int
     return lo;
 }
 
-template <typename T, int MAXP, int FORMED>
-__global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
+template <typename T, int MAXP, int FORMED, int SZ, int NT>
+__global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
 {
     constexpr int NV = NG * MAXP;
     constexpr int LV = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : 0;
     static_assert(LV > 0, "k_projs: NG * MAXP must be 8, 16 or 32");
     constexpr unsigned TS = sizeof(T);
-    constexpr int RPI = 3;   // rounds of 64 box points per U load batch (w <= 13 in one batch)
-    constexpr int PPT = 3;   // staged points per thread per load batch
+    constexpr int RPI = SZ >= 4 ? 2 : 3;   // rounds of 64 box points per U load batch
+    constexpr int PPT = SZ >= 4 ? 2 : 3;   // staged points per thread per load batch
     const Geo &G = A.g;
     const int q = blockIdx.x;
     const int b0 = q * NG, gn = min(NG, A.B - b0);
@@ -767,15 +768,13 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
     }
     const int cnt = A.glist_cnt[q];
     if (nact == 0 || cnt == 0) return;
-    // LDS: per-image totals [cnt][NV] | image data [cnt] | staged plane [pu][NG]
+    // LDS: image data [cnt] | staged slab [SZ][pu][NG]
     __shared__ int sh_ncmax, sh_lo, sh_hi;
     if (tid == 0) sh_ncmax = 0;
     __syncthreads();
     extern __shared__ unsigned char smem_raw[];
-    double *sacc = reinterpret_cast<double *>(smem_raw);
-    ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw + (size_t)cnt * NV * sizeof(double));
-    T *pl = reinterpret_cast<T *>(smem_raw + (size_t)cnt * NV * sizeof(double) + (size_t)cnt * sizeof(ImgMeta));
-    for (int i = tid; i < cnt * NV; i += blockDim.x) sacc[i] = 0.0;
+    ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw);
+    T *pl = reinterpret_cast<T *>(smem_raw + ((size_t)cnt * sizeof(ImgMeta) + 15) / 16 * 16);
     for (int li = tid; li < cnt; li += blockDim.x) {
         const int J = A.glist[(size_t)q * GLCAP + li];
         const int t = A.img_type[J], nc = A.nlocc[t];
@@ -785,6 +784,11 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
         e.uoff = A.udv_off[J];
         im[li] = e;
         atomicMax(&sh_ncmax, nc);
+        // this group's projections start at zero; each slab adds its totals
+#pragma unroll
+        for (int m = 0; m < NG; ++m)
+            if ((onmask >> m) & 1u)
+                for (int i = 0; i < e.np; ++i) A.s[(size_t)(b0 + m) * A.n_ip + e.ipb + i] = 0.0;
     }
     const size_t gb = (size_t)gn * M3 * TS;
     const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
@@ -805,29 +809,37 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
             }
         }
         if (ux1 < ux0) continue;
-        const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, up = ux * uy;
+        const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, up = ux * uy, sp = SZ * up;
+        // slabs of SZ planes whose first global z is a multiple of SZ (the same
+        // slab boundaries for every grouping of the nodes)
+        const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
 #pragma unroll 1
-        for (int gz = uz0; gz <= uz1; ++gz) {
-            __syncthreads();   // the previous plane is consumed (and sacc zeroed)
-            // the images that can cross plane gz form one range of the z-sorted list
+        for (int gzs = zs0; gzs <= uz1; gzs += SZ) {
+            // the previous slab is consumed, and this wave's atomics (and the
+            // zeroing stores) have been performed in L2 (vmcnt 0) before any
+            // wave adds the next slab's totals
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            // the images that can cross the slab form one range of the z-sorted list
             if (tid == 0) {
-                sh_lo = first_cz_at_least(im, cnt, gz - sh_ncmax);
-                sh_hi = first_cz_at_least(im, cnt, gz + sh_ncmax + 1);
+                sh_lo = first_cz_at_least(im, cnt, gzs - sh_ncmax);
+                sh_hi = first_cz_at_least(im, cnt, gzs + SZ + sh_ncmax);
             }
-            // stage the plane: every load of the thread's points issued before
-            // the first store (one memory latency per plane)
+            // stage the slab ([plane][point][member]): every load of the
+            // thread's points issued before the first store
 #pragma unroll 1
-            for (int p0 = 0; p0 < up; p0 += PPT * (int)blockDim.x) {
+            for (int p0 = 0; p0 < sp; p0 += PPT * (int)blockDim.x) {
                 T xv[PPT][NG], yv[PPT][NG];
 #pragma unroll
                 for (int r = 0; r < PPT; ++r) {
                     const int p = p0 + r * (int)blockDim.x + tid;
-                    const int yy = p / ux, xx = p - yy * ux;
-                    const int gx = ux0 + xx, gy = uy0 + yy;
+                    const int sl = p / up, pq = p - sl * up;
+                    const int yy = pq / ux, xx = pq - yy * ux;
+                    const int gx = ux0 + xx, gy = uy0 + yy, gz = gzs + sl;
 #pragma unroll
                     for (int m = 0; m < NG; ++m) {
                         const int lx = gx - ni[m] + nloc, ly = gy - nj[m] + nloc, lz = gz - nk[m] + nloc;
-                        const bool ok = p < up && in_sub[m] && (unsigned)lx < (unsigned)M &&
+                        const bool ok = p < sp && in_sub[m] && (unsigned)lx < (unsigned)M &&
                                         (unsigned)ly < (unsigned)M && (unsigned)lz < (unsigned)M;
                         const unsigned o = ok ? (unsigned)(m * M3 + lz * M2 + ly * M + lx) * TS : OOB;
                         xv[r][m] = bload<T>(rX, o);
@@ -837,7 +849,7 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
 #pragma unroll
                 for (int r = 0; r < PPT; ++r) {
                     const int p = p0 + r * (int)blockDim.x + tid;
-                    if (p < up) {
+                    if (p < sp) {
 #pragma unroll
                         for (int m = 0; m < NG; ++m)
                             pl[(size_t)p * NG + m] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
@@ -851,8 +863,8 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
                 // the image's data is wave-uniform: keep it in scalar registers
                 const ImgMeta &e = im[li];
                 const int w = __builtin_amdgcn_readfirstlane(e.w), oz = __builtin_amdgcn_readfirstlane(e.oz);
-                const int bz = gz - oz;
-                if ((unsigned)bz >= (unsigned)w) continue;
+                const int bz0 = gzs - oz;   // box z of the slab's first plane
+                if (bz0 + SZ <= 0 || bz0 >= w) continue;
                 const int ox = __builtin_amdgcn_readfirstlane(e.ox), oy = __builtin_amdgcn_readfirstlane(e.oy);
                 const int rlo = max(0, uy0 - oy), rhi = min(w - 1, uy1 - oy);
                 const int clo = max(0, ux0 - ox), chi = min(w - 1, ux1 - ox);
@@ -861,51 +873,62 @@ __global__ void __launch_bounds__(projs_threads<MAXP>()) k_projs(PArgs<T> A)
                 const long uoff = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(e.uoff >> 32)) << 32) |
                                          (unsigned)__builtin_amdgcn_readfirstlane((int)e.uoff));
                 const __amdgpu_buffer_rsrc_t rU = brsrc(A.udv + uoff, (size_t)np * w3 * TS);
+                // the value this lane adds at the end: member kw / MAXP, projector kw % MAXP
+                const int kw = lane >> (6 - LV), mw = kw / MAXP, iw = kw - mw * MAXP;
+                const bool writer = (lane & ((64 >> LV) - 1)) == 0 && ((onmask >> mw) & 1u) && iw < np;
+                const int t = __builtin_amdgcn_readfirstlane(e.t), ipb = __builtin_amdgcn_readfirstlane(e.ipb);
+                const double idn = writer ? A.inv_denom[16 * t + iw] : 0.0;
                 double acc[NV];
 #pragma unroll
                 for (int k = 0; k < NV; ++k) acc[k] = 0.0;
                 const int a_hi = (rhi + 1) * w;
+                // a / w through the float reciprocal: exact, since (a + 1/2)/w
+                // stays at least 1/(2w) away from an integer (a < w^2, w < 1000)
+                const float rw = 1.0f / (float)w;
                 // RPI rounds of 64 box points at a time, their U loads issued together
 #pragma unroll 1
                 for (int c0 = (rlo * w) & ~63; c0 < a_hi; c0 += 64 * RPI) {
-                    T u[RPI][MAXP];
+                    T u[RPI][SZ][MAXP];
                     int pp[RPI];
                     bool in[RPI];
 #pragma unroll
                     for (int r = 0; r < RPI; ++r) {
                         const int a = c0 + 64 * r + lane;
-                        const int by = a / w, bx = a - by * w;
+                        const int by = (int)(((float)a + 0.5f) * rw), bx = a - by * w;
                         in[r] = a < a_hi && by >= rlo && by <= rhi && bx >= clo && bx <= chi;
 #pragma unroll
-                        for (int i = 0; i < MAXP; ++i)
-                            u[r][i] = bload<T>(rU, (in[r] && i < np) ? (unsigned)(i * w3 + bz * w2 + a) * TS : OOB);
+                        for (int z = 0; z < SZ; ++z) {
+                            const bool okz = in[r] && (unsigned)(bz0 + z) < (unsigned)w;
+#pragma unroll
+                            for (int i = 0; i < MAXP; ++i)
+                                u[r][z][i] = bload<T>(rU, (okz && i < np) ? (unsigned)(i * w3 + (bz0 + z) * w2 + a) * TS : OOB);
+                        }
                         pp[r] = in[r] ? (oy + by - uy0) * ux + (ox + bx - ux0) : 0;
                     }
 #pragma unroll
                     for (int r = 0; r < RPI; ++r) {
-                        const T *pv = pl + (size_t)pp[r] * NG;
-                        T v[NG];
 #pragma unroll
-                        for (int m = 0; m < NG; ++m) v[m] = pv[m];
-                        if (in[r]) {
+                        for (int z = 0; z < SZ; ++z) {
+                            const T *pv = pl + ((size_t)z * up + pp[r]) * NG;
+                            T v[NG];
 #pragma unroll
-                            for (int m = 0; m < NG; ++m)
+                            for (int m = 0; m < NG; ++m) v[m] = pv[m];
+                            if (in[r]) {
 #pragma unroll
-                                for (int i = 0; i < MAXP; ++i)
-                                    acc[m * MAXP + i] += (double)v[m] * (double)u[r][i];
+                                for (int m = 0; m < NG; ++m)
+#pragma unroll
+                                    for (int i = 0; i < MAXP; ++i)
+                                        acc[m * MAXP + i] += (double)v[m] * (double)u[r][z][i];
+                            }
                         }
                     }
                 }
                 const double tot = wave_sums<NV>(acc, lane);
-                if ((lane & ((64 >> LV) - 1)) == 0) sacc[(size_t)li * NV + (lane >> (6 - LV))] += tot;
+                // one lane per value and slab; slabs are ordered by the fence
+                // and barrier above, so the additions happen in slab order
+                if (writer) unsafeAtomicAdd(A.s + (size_t)(b0 + mw) * A.n_ip + ipb + iw, tot * idn);
             }
         }
-    }
-    __syncthreads();
-    for (int e = tid; e < cnt * NV; e += blockDim.x) {
-        const int li = e / NV, k = e - li * NV, m = k / MAXP, i = k - m * MAXP;
-        if (((onmask >> m) & 1u) && i < im[li].np)
-            A.s[(size_t)(b0 + m) * A.n_ip + im[li].ipb + i] = sacc[e] * A.inv_denom[16 * im[li].t + i];
     }
 }
 

@@ -270,7 +270,7 @@ int node_images(sqg_ctx *c, int B)
 size_t nlapplyg_lds(const sqg_ctx *c)
 {
     const size_t head = ((size_t)(std::max(c->n_ip, 1) + c->gl_max + 1) * 4 + 15) / 16 * 16;
-    return head + (size_t)std::max(c->gl_max * c->max_nproj, 1) * NG * sizeof(double);
+    return head + (size_t)std::max(c->gl_max * c->max_nproj, 1) * SLW * sizeof(double);
 }
 
 template <typename T, int K>

@@ -23,6 +23,8 @@ constexpr int FDN = 6;          // half FD order (12th-order stencil)
 constexpr int WIN = 2 * FDN + 1;
 constexpr int LCAP = 512;       // max images meeting one node cube
 constexpr int NG = 8;           // node cubes per workgroup of the grouped projector kernels
+constexpr int SLW = 10;         // LDS row of k_nlapplyg: NG projections padded to 80 B (rows
+                                // r and r + 4 would share banks at 64 B)
 constexpr int GLCAP = 1024;     // max images meeting the cubes of one group
 constexpr int MAXM = 43;        // largest cube side one workgroup marches ((M+1)/2*M <= 1024)
 constexpr int MAXIT = 500;      // Lanczos max_iter (sq.cpp:967)
@@ -951,7 +953,7 @@ struct NGArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(512) k_nlapplyg(NGArgs<T> A)
+__global__ void __launch_bounds__(512, 4) k_nlapplyg(NGArgs<T> A)
 {
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
@@ -975,7 +977,7 @@ __global__ void __launch_bounds__(512) k_nlapplyg(NGArgs<T> A)
         }
     }
     if (nact == 0) return;
-    // LDS: loc [n_ip] | row offsets [cnt + 1] | sl [rows][NG]
+    // LDS: loc [n_ip] | row offsets [cnt + 1] | sl [rows][SLW]
     extern __shared__ unsigned char smem_raw[];
     int *loc = reinterpret_cast<int *>(smem_raw);
     const int cnt = A.glist_cnt[q];
@@ -995,10 +997,10 @@ __global__ void __launch_bounds__(512) k_nlapplyg(NGArgs<T> A)
             loc[ipb + i] = r0 + i;
 #pragma unroll
             for (int m = 0; m < NG; ++m)
-                sl[(size_t)(r0 + i) * NG + m] = on[m] ? A.nl.s[(size_t)(b0 + m) * A.nl.n_ip + ipb + i] : 0.0;
+                sl[(size_t)(r0 + i) * SLW + m] = on[m] ? A.nl.s[(size_t)(b0 + m) * A.nl.n_ip + ipb + i] : 0.0;
         }
     }
-    if (cnt == 0 && threadIdx.x < NG) sl[threadIdx.x] = 0.0;
+    if (cnt == 0 && threadIdx.x < SLW) sl[threadIdx.x] = 0.0;
     __syncthreads();
     const int E3 = (int)A.nl.E3;
     const int KA = A.nl.k_alloc;
@@ -1028,7 +1030,7 @@ __global__ void __launch_bounds__(512) k_nlapplyg(NGArgs<T> A)
             }
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                const double *row = sl + (size_t)loc[ip[c]] * NG;
+                const double *row = sl + (size_t)loc[ip[c]] * SLW;
 #pragma unroll
                 for (int m = 0; m < NG; ++m) a[m] += row[m] * (double)u[c];
             }

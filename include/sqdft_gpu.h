@@ -159,6 +159,28 @@ int sqg_adjoint_veff(sqg_ctx *ctx, const double *w, int fixed_n, double *dLdV);
 /* drho (n^3 out) along dV (n^3): forward mode through the recurrence. */
 int sqg_tangent_veff(sqg_ctx *ctx, const double *dV, int fixed_n, double *drho);
 
+/* ---- The SCF step around the SQ call (FP64 on the device) ----
+ * AAJ Poisson solve <- PoissonSolver_AAJ(DS_SQ*) (src/poisson.cpp:19-325,
+ * inc/func_sq.h:303) with PoissonResidual (src/poisson.cpp:715-928) and
+ * AndersonExtrapolation (src/anderson.cpp:20-163): phi of
+ * -Lap_FD phi = 4 pi rhs on the periodic n^3 grid (rhs = rho + b, x fastest)
+ * from phi_guess, Jacobi relaxation beta, an Anderson step every p
+ * iterations over the last m (1..8) differences, stopped at relative
+ * residual <= tol or maxiter; phi is shifted to phi(0,0,0) = 0.  iters and
+ * relres are the counts the reference prints. */
+int sqg_poisson_aaj(sqg_ctx *ctx, const double *rhs, const double *phi_guess, double tol, int maxiter,
+                    double beta, int m, int p, double *phi, int *iters, double *relres);
+
+/* LDA exchange-correlation potential <- ExchangeCorrelationPotential(DS_SQ*)
+ * (src/scf.cpp:19-64, inc/func_sq.h:309): n^3 values of rho in, Vxc out. */
+int sqg_vxc_lda(sqg_ctx *ctx, const double *rho, double *vxc);
+
+/* Total energy per atom <- EvaluateTotalEnergy(DS_SQ*) (src/energy.cpp:19-87,
+ * inc/func_sq.h:521): phi and vxc of the potential the SQ call used, rho its
+ * output, ebs/eent from sqg_density, eself/ecorr from the pseudocharge. */
+int sqg_total_energy(sqg_ctx *ctx, const double *phi, const double *b, const double *rho, const double *vxc,
+                     double ebs, double eent, double eself, double ecorr, int n_atm, double *etot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,7 @@ typedef struct sqdft_type_spec {
     const char *psd_path;    /* Troullier-Martins table in the `atom` format */
     int natm;                /* atoms of this type */
     const double *xyz;       /* 3*natm Cartesian coordinates (bohr), x,y,z per atom */
+    double rb;               /* pseudocharge radius (".atoms" rb); 0 -> no pseudocharge */
 } sqdft_type_spec;
 
 /* Grid and SQ parameters (".input" keywords, readfiles.cpp:86-316). */
@@ -104,6 +105,27 @@ int  sqdft_system_images(const sqdft_system *S, double *xyz, int *type, int *ato
 /* Projector tables U_l Y_lm (V_l - V_loc) on each image's (2 nlocc+1)^3 cube,
  * projector-major then z,y,x (udv_len doubles). */
 int  sqdft_system_udv(const sqdft_system *S, double *udv);
+
+/* Pseudocharge and starting density of the SCF (ChargeDensity,
+ * src/initialize.cpp:471-784, over the rb images of Replica_atoms,
+ * src/initialize.cpp:21-150; first MD step, Correction 1).  Outputs, any of
+ * which may be NULL: b[n^3] (x fastest), rho[n^3] = rho_at scaled to hold
+ * -Ncharge electrons, ncharge = sum b h^3 (negative), eself, and ecorr,
+ * the energy correction (the reference prints it divided by the atom count).
+ * Returns SQDFT_E_ARG when a type has rb <= 0. */
+int  sqdft_system_pseudocharge(const sqdft_system *S, double *b, double *rho, double *ncharge,
+                               double *eself, double *ecorr);
+
+/* Poisson-solver and SCF-mixing parameters (".input" poisson_tol,
+ * poisson_maxiter, beta_aaj, m_aaj, p_aaj; defaults 1e-6, 1000, 0.6, 7, 6
+ * for systems built from a spec). */
+typedef struct sqdft_poisson_params {
+    double tol;
+    int maxiter;
+    double beta;
+    int m, p;
+} sqdft_poisson_params;
+int  sqdft_system_poisson_params(const sqdft_system *S, sqdft_poisson_params *pp);
 /* Cubic-spline evaluation with the reference's routine (tests). */
 int  sqdft_spline_eval(const double *X, const double *Y, const double *YD, int len,
                        const double *x, double *y, int nx);

@@ -90,6 +90,11 @@ def load() -> C.CDLL:
                                      c_dbl_p, c_dbl_p]),
         "sqo_forces_nonlocal": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_int_p, C.c_int, c_dbl_p,
                                           C.c_int]),
+        "sqo_poisson_aaj": (C.c_int, [C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_int,
+                                      C.c_double, C.c_int, C.c_int, c_dbl_p, c_int_p, c_dbl_p]),
+        "sqo_vxc_lda": (None, [C.c_size_t, c_dbl_p, c_dbl_p]),
+        "sqo_total_energy": (C.c_double, [C.c_size_t, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_int]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -291,3 +296,31 @@ def fd_coeffs(FDn, h):
     lap, grad = np.zeros(16), np.zeros(16)
     load().sqo_fd_coeffs(FDn, h, _d(lap), _d(grad))
     return lap[: FDn + 1], grad[: FDn + 1]
+
+
+def poisson_aaj(n, FDn, lap, rhs, phi_guess, tol=1e-6, maxiter=1000, beta=0.6, m=7, p=6):
+    """AAJ solve of -Lap_FD phi = 4 pi rhs (poisson.cpp:19-325): (phi, iters, relres)."""
+    lp = np.ascontiguousarray(np.asarray(lap, dtype=np.float64))
+    r = np.ascontiguousarray(np.asarray(rhs, dtype=np.float64).reshape(-1))
+    g = np.ascontiguousarray(np.asarray(phi_guess, dtype=np.float64).reshape(-1))
+    phi = np.zeros(n ** 3)
+    it, rr = C.c_int(), C.c_double()
+    st = load().sqo_poisson_aaj(n, FDn, _d(lp), _d(r), _d(g), tol, maxiter, beta, m, p, _d(phi), C.byref(it),
+                                C.byref(rr))
+    if st < 0:
+        raise RuntimeError(f"sqo_poisson_aaj: {st}")
+    return phi, it.value, rr.value
+
+
+def vxc_lda(rho):
+    """LDA exchange-correlation potential of scf.cpp:19-64."""
+    r = np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+    v = np.zeros_like(r)
+    load().sqo_vxc_lda(r.size, _d(r), _d(v))
+    return v
+
+
+def total_energy(h, phi, b, rho, vxc, ebs, eent, eself, ecorr, n_atm):
+    """Total energy per atom of EvaluateTotalEnergy (energy.cpp:19-87)."""
+    a = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1)) for x in (phi, b, rho, vxc)]
+    return load().sqo_total_energy(a[0].size, h, *[_d(x) for x in a], ebs, eent, eself, ecorr, n_atm)

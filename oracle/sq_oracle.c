@@ -1360,3 +1360,206 @@ int sqo_forces_nonlocal(const sqo_sys *S, const double *veff, const double *chi,
     free(part);
     return st;
 }
+
+/* ------------------------------------------------------------------ */
+/* Poisson: AAJ (Jacobi + periodic Anderson), poisson.cpp:19-325,     */
+/* PoissonResidual poisson.cpp:715-928, AndersonExtrapolation and     */
+/* PseudoInverseTimesVec anderson.cpp:20-163                          */
+/* ------------------------------------------------------------------ */
+
+/* x = pinv(A) b for the symmetric m x m matrix A (column major), by a
+ * one-sided Jacobi SVD of A; singular values that come out exactly zero
+ * are skipped, as anderson.cpp:134-142 skips w_j == 0 (a zero history
+ * column gives an exactly zero row and column). */
+static void pinv_times_vec(const double *Ac, const double *b, double *x, int m)
+{
+    double U[64], V[64], w[8];
+    for (int i = 0; i < m * m; i++) U[i] = Ac[i];   /* columns of U start as columns of A */
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) V[j * m + i] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                double a = 0, bb = 0, c = 0;
+                for (int i = 0; i < m; i++) {
+                    a += U[p * m + i] * U[p * m + i];
+                    bb += U[q * m + i] * U[q * m + i];
+                    c += U[p * m + i] * U[q * m + i];
+                }
+                if (c == 0.0) continue;
+                const double rel = fabs(c) / sqrt(a * bb);
+                if (rel > off) off = rel;
+                const double zeta = (bb - a) / (2 * c);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+                const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+                for (int i = 0; i < m; i++) {
+                    const double up = U[p * m + i], uq = U[q * m + i];
+                    U[p * m + i] = cs * up - sn * uq;
+                    U[q * m + i] = sn * up + cs * uq;
+                    const double vp = V[p * m + i], vq = V[q * m + i];
+                    V[p * m + i] = cs * vp - sn * vq;
+                    V[q * m + i] = sn * vp + cs * vq;
+                }
+            }
+        if (off < 1e-15) break;
+    }
+    for (int j = 0; j < m; j++) {
+        double s = 0;
+        for (int i = 0; i < m; i++) s += U[j * m + i] * U[j * m + i];
+        w[j] = sqrt(s);
+    }
+    double tmp[8];
+    for (int j = 0; j < m; j++) {   /* (U_j/w_j)' b / w_j */
+        double s = 0.0;
+        if (w[j] != 0.0) {
+            for (int i = 0; i < m; i++) s += U[j * m + i] / w[j] * b[i];
+            s /= w[j];
+        }
+        tmp[j] = s;
+    }
+    for (int i = 0; i < m; i++) {
+        double s = 0.0;
+        for (int j = 0; j < m; j++) s += V[j * m + i] * tmp[j];
+        x[i] = s;
+    }
+}
+
+/* Jacobi residual of -Lap phi = 4 pi rhs on the periodic n^3 grid:
+ * r = -(4 pi rhs + Lap_FD phi)/(3 lap[0]) (poisson.cpp:779-786). */
+static void aaj_residual(int n, int FDn, const double *lap, const double *phi, const double *rhs, double *r)
+{
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                const size_t c = ((size_t)k * n + j) * n + i;
+                double L = phi[c] * 3 * lap[0];
+                for (int a = 1; a <= FDn; a++)
+                    L += (phi[((size_t)k * n + j) * n + wrapi(i - a, n)] + phi[((size_t)k * n + j) * n + wrapi(i + a, n)] +
+                          phi[((size_t)k * n + wrapi(j - a, n)) * n + i] + phi[((size_t)k * n + wrapi(j + a, n)) * n + i] +
+                          phi[((size_t)wrapi(k - a, n) * n + j) * n + i] + phi[((size_t)wrapi(k + a, n) * n + j) * n + i]) *
+                         lap[a];
+                r[c] = -(((4 * M_PI) * (rhs[c]) + L) / (3 * lap[0]));
+            }
+}
+
+int sqo_poisson_aaj(int n, int FDn, const double *lap, const double *rhs, const double *phi_guess, double tol,
+                    int maxiter, double beta, int m, int p, double *phi, int *iters, double *relres)
+{
+    if (m < 1 || m > 8 || p < 1) return -1;
+    const size_t N = (size_t)n * n * n;
+    double *old = malloc(N * sizeof(double)), *nw = malloc(N * sizeof(double)), *res = malloc(N * sizeof(double));
+    double *Xold = calloc(N, sizeof(double)), *Fold = calloc(N, sizeof(double)), *am = malloc(N * sizeof(double));
+    double *DX = calloc(N * m, sizeof(double)), *DF = calloc(N * m, sizeof(double));
+    if (!old || !nw || !res || !Xold || !Fold || !am || !DX || !DF) return -2;
+    memcpy(old, phi_guess, N * sizeof(double));
+    double rn = 0;
+    for (size_t c = 0; c < N; c++) rn = rn + rhs[c] * rhs[c];
+    rn = sqrt(rn);
+    rn *= 4 * M_PI * fabs(-1 / (3 * lap[0]));
+    int iter = 1;
+    double r = tol + 1;
+    while (r > tol && iter <= maxiter) {
+        aaj_residual(n, FDn, lap, old, rhs, res);
+        for (size_t c = 0; c < N; c++) nw[c] = old[c] + beta * res[c];
+        double s = 0;
+        for (size_t c = 0; c < N; c++) s = s + res[c] * res[c];
+        r = sqrt(s) / rn;
+        if (iter > 1) {
+            const int k = ((iter - 2) % m);
+            for (size_t c = 0; c < N; c++) {
+                DX[k * N + c] = old[c] - Xold[c];
+                DF[k * N + c] = res[c] - Fold[c];
+            }
+        }
+        memcpy(Xold, old, N * sizeof(double));
+        memcpy(Fold, res, N * sizeof(double));
+        if (iter % p == 0 && iter > 1) {
+            double FtF[64], Ftf[8], x[8];
+            int ctr = 0;
+            for (int jj = 0; jj < m; jj++)
+                for (int ii = 0; ii < m; ii++) {
+                    double t = 0;
+                    for (size_t c = 0; c < N; c++) t = t + DF[ii * N + c] * DF[jj * N + c];
+                    FtF[ctr++] = t;
+                }
+            for (int jj = 0; jj < m; jj++) {
+                double t = 0;
+                for (size_t c = 0; c < N; c++) t = t + DF[jj * N + c] * res[c];
+                Ftf[jj] = t;
+            }
+            pinv_times_vec(FtF, Ftf, x, m);
+            for (size_t c = 0; c < N; c++) {
+                double t = 0;
+                for (int jj = 0; jj < m; jj++) t = t + (DX[jj * N + c] + beta * DF[jj * N + c]) * x[jj];
+                am[c] = t;
+            }
+            for (size_t c = 0; c < N; c++) nw[c] = nw[c] - am[c];
+        }
+        if (r <= tol || iter == maxiter) memcpy(phi, old, N * sizeof(double));
+        memcpy(old, nw, N * sizeof(double));
+        iter = iter + 1;
+    }
+    const double p000 = phi[0];   /* phi is defined up to a constant: phi(0,0,0) = 0 (poisson.cpp:272-290) */
+    for (size_t c = 0; c < N; c++) phi[c] = phi[c] + 0.0 - p000;
+    *iters = iter - 1;
+    *relres = r;
+    free(old); free(nw); free(res); free(Xold); free(Fold); free(am); free(DX); free(DF);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* LDA exchange-correlation potential (Perdew-Wang 92 correlation,     */
+/* Slater exchange), scf.cpp:19-64                                     */
+/* ------------------------------------------------------------------ */
+void sqo_vxc_lda(size_t N, const double *rho, double *vxc)
+{
+    const double p = 1.0, A = 0.031091, alpha1 = 0.21370, beta1 = 7.5957, beta2 = 3.5876, beta3 = 1.6382,
+                 beta4 = 0.49294, C3 = 0.9847450218427;
+    for (size_t c = 0; c < N; c++) {
+        const double rhoi = rho[c];
+        double V;
+        if (rhoi == 0) {
+            V = 0.0;
+        } else {
+            const double rs = pow((0.75 / (M_PI * rhoi)), (1.0 / 3.0));
+            const double G = 2.0 * A * (beta1 * pow(rs, 0.5) + beta2 * rs + beta3 * pow(rs, 1.5) + beta4 * pow(rs, (p + 1.0)));
+            const double dG = A * (beta1 * pow(rs, -0.5) + 2.0 * beta2 + 3.0 * beta3 * pow(rs, 0.5) +
+                                   2.0 * (p + 1.0) * beta4 * pow(rs, p));
+            V = (-2.0 * A * (1.0 + alpha1 * rs)) * log(1.0 + 1.0 / G) -
+                (rs / 3.0) * (-2.0 * A * alpha1 * log(1.0 + 1.0 / G) - ((-2.0 * A * (1.0 + alpha1 * rs)) * dG) / (G * G + G));
+        }
+        vxc[c] = V - C3 * pow(rhoi, 1.0 / 3.0);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Total energy per atom: EvaluateTotalEnergy, energy.cpp:19-87        */
+/* (phi and vxc of the potential the SQ call used, rho its output)     */
+/* ------------------------------------------------------------------ */
+double sqo_total_energy(size_t N, double h, const double *phi, const double *b, const double *rho,
+                        const double *vxc, double Ebs, double Eent, double Eself, double Ecorr, int n_atm)
+{
+    const double p = 1.0, A = 0.031091, alpha1 = 0.21370, beta1 = 7.5957, beta2 = 3.5876, beta3 = 1.6382,
+                 beta4 = 0.49294, C2 = 0.73855876638202;
+    double Eele = 0.0, Exc = 0.0, Exc_del = 0.0;
+    for (size_t c = 0; c < N; c++) Eele = Eele + 0.5 * (phi[c] * (b[c] - rho[c])) * h * h * h;
+    for (size_t c = 0; c < N; c++) {
+        const double rhoi = rho[c];
+        const double Ex = -C2 * pow(rhoi, 4.0 / 3.0);
+        double Ec;
+        if (rhoi == 0) {
+            Ec = 0.0;
+        } else {
+            Ec = pow(0.75 / (M_PI * rhoi), (1.0 / 3.0));
+            Ec = -2.0 * A * (1.0 + alpha1 * Ec) *
+                 log(1.0 + 1.0 / (2.0 * A * (beta1 * pow(Ec, 0.5) + beta2 * Ec + beta3 * pow(Ec, 1.5) + beta4 * pow(Ec, (p + 1.0)))));
+        }
+        Exc = Exc + (Ec * rhoi + Ex) * h * h * h;
+        Exc_del = Exc_del + -1.0 * vxc[c] * rhoi * h * h * h;
+    }
+    double Etot = Eele + Exc + Exc_del;
+    Etot = Etot - Eself;
+    Etot = Etot + Ebs + Eent + Ecorr;
+    return Etot / n_atm;
+}

@@ -13,6 +13,7 @@
  */
 #ifndef SQ_ORACLE_H
 #define SQ_ORACLE_H
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -176,6 +177,17 @@ int sqo_forces_nonlocal(const sqo_sys *S, const double *veff, const double *chi,
                         const double *zee, const double *Ci, const int *nodes, int n_nodes,
                         double *f, int nthreads);
 
+
+/* AAJ Poisson solver (poisson.cpp:19-325 with anderson.cpp:20-163): phi of
+ * -Lap_FD phi = 4 pi rhs on the periodic n^3 grid from phi_guess, shifted so
+ * that phi(0,0,0) = 0; iters and the final relative residual out. */
+int sqo_poisson_aaj(int n, int FDn, const double *lap, const double *rhs, const double *phi_guess, double tol,
+                    int maxiter, double beta, int m, int p, double *phi, int *iters, double *relres);
+/* LDA exchange-correlation potential of scf.cpp:19-64 (N grid values). */
+void sqo_vxc_lda(size_t N, const double *rho, double *vxc);
+/* Total energy per atom of EvaluateTotalEnergy (energy.cpp:19-87). */
+double sqo_total_energy(size_t N, double h, const double *phi, const double *b, const double *rho,
+                        const double *vxc, double Ebs, double Eent, double Eself, double Ecorr, int n_atm);
 #ifdef __cplusplus
 }
 #endif

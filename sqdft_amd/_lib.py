@@ -18,7 +18,12 @@ c_long_p = C.POINTER(C.c_long)
 
 class TypeSpec(C.Structure):
     _fields_ = [("symbol", C.c_char_p), ("Z", C.c_int), ("lloc", C.c_int),
-                ("psd_path", C.c_char_p), ("natm", C.c_int), ("xyz", c_dbl_p)]
+                ("psd_path", C.c_char_p), ("natm", C.c_int), ("xyz", c_dbl_p), ("rb", C.c_double)]
+
+
+class PoissonParams(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("beta", C.c_double),
+                ("m", C.c_int), ("p", C.c_int)]
 
 
 class Spec(C.Structure):
@@ -75,6 +80,13 @@ SIGNATURES = {
     "sqdft_system_atoms": (C.c_int, [C.c_void_p, c_dbl_p, c_int_p]),
     "sqdft_system_images": (C.c_int, [C.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p, c_long_p]),
     "sqdft_system_udv": (C.c_int, [C.c_void_p, c_dbl_p]),
+    "sqg_poisson_aaj": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                  c_dbl_p, c_int_p, c_dbl_p]),
+    "sqg_vxc_lda": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p]),
+    "sqg_total_energy": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, c_dbl_p]),
+    "sqdft_system_pseudocharge": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "sqdft_system_poisson_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sqdft_spline_eval": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, C.c_int]),
     "sqdft_spline_derivs": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int]),
     # sqdft_gpu.h

@@ -71,12 +71,12 @@ def build(name: str, seed: int = 0, psd_dir: str = PSD_DIR, **over) -> System:
     ncell, n_cell = c["ncell"], c["n_cell"]
     if c["kind"] == "al":
         a = 7.78
-        types = [AtomType("Al", 3, 1, os.path.join(psd_dir, "psd_Al.pot"), fcc_al(ncell, a, 0.1, seed))]
+        types = [AtomType("Al", 3, 1, os.path.join(psd_dir, "psd_Al.pot"), fcc_al(ncell, a, 0.1, seed), rb=6.0)]
     else:
         a = 7.6
         li, h = rocksalt_lih(ncell, a, 0.05, seed)
-        types = [AtomType("Li", 1, 0, os.path.join(psd_dir, "psd_Li.pot"), li),
-                 AtomType("H", 1, 0, os.path.join(psd_dir, "psd_H.pot"), h)]
+        types = [AtomType("Li", 1, 0, os.path.join(psd_dir, "psd_Li.pot"), li, rb=8.0),
+                 AtomType("H", 1, 0, os.path.join(psd_dir, "psd_H.pot"), h, rb=8.0)]
     # the mesh is computed per unit cell, as the reference does (readfiles.cpp:294-303)
     return System.build(n=n_cell * ncell, L_box=a * ncell, types=types, Rcut=c["Rcut"], npl=c["npl"],
                         T=c["T"], h=a / n_cell)

@@ -10,6 +10,7 @@
 #include "sqdft_system.h"
 #include "sq_kernels.hpp"
 #include "sq_adjoint_kernels.hpp"
+#include "sq_poisson_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -83,6 +84,8 @@ struct sqg_ctx {
     // Veff derivatives: vectors, per-node sign / d rho/d lambda / weight,
     // n^3 input and output, per-step tangent records
     DBuf adj, adj_sgn, adj_dl, adj_wt, adj_in, adj_out, adj_rec;
+    // AAJ Poisson: iterates, residual, history, rhs, result, block partials
+    DBuf pz_phi, pz_new, pz_res, pz_X, pz_F, pz_rhs, pz_out, pz_DX, pz_DF, pz_part;
     bool have_state = false;
     double lambda_f = 0.0;
     // device timing of the hot kernels over the last density call (which:
@@ -819,6 +822,7 @@ int rho_assembly(sqg_ctx *c, double lf, double *rho, double *Ebs, double *Eent, 
 }
 
 #include "sq_adjoint_host.hpp"
+#include "sq_poisson_host.hpp"
 
 // Event bookkeeping of one call: reset before its first launch, summed after
 // its stream has synchronised.
@@ -1360,6 +1364,59 @@ int sqg_tangent_veff(sqg_ctx *c, const double *dV, int fixed_n, double *drho)
         const double dlam = -num / den;
         for (int i = 0; i < nn; ++i) drho[i] += sgn[i] * dl[i] * dlam;
     }
+    return SQG_OK;
+}
+
+int sqg_poisson_aaj(sqg_ctx *c, const double *rhs, const double *phi_guess, double tol, int maxiter, double beta,
+                    int m, int p, double *phi, int *iters, double *relres)
+{
+    // periodic stencil indices wrap once: the grid must span the stencil half-width
+    if (!c || !rhs || !phi_guess || !phi || m < 1 || m > 8 || p < 1 || maxiter < 1 || !(tol > 0) || c->p.n < FDN)
+        return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    return poisson_aaj(c, rhs, phi_guess, tol, maxiter, beta, m, p, phi, iters, relres);
+}
+
+int sqg_vxc_lda(sqg_ctx *c, const double *rho, double *vxc)
+{
+    if (!c || !rho || !vxc) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    const size_t N = (size_t)c->p.n * c->p.n * c->p.n;
+    HIPCHK(c->pz_rhs.ensure(N * sizeof(double)));
+    HIPCHK(c->pz_out.ensure(N * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->pz_rhs.p, rho, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_vxc_lda, dim3((int)std::min<size_t>((N + PB - 1) / PB, 1024)), dim3(PB), 0, c->st,
+                       c->pz_rhs.as<double>(), N, c->pz_out.as<double>());
+    HIPCHK(hipMemcpyAsync(vxc, c->pz_out.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SQG_OK;
+}
+
+int sqg_total_energy(sqg_ctx *c, const double *phi, const double *b, const double *rho, const double *vxc,
+                     double ebs, double eent, double eself, double ecorr, int n_atm, double *etot)
+{
+    if (!c || !phi || !b || !rho || !vxc || !etot || n_atm < 1) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    const size_t N = (size_t)c->p.n * c->p.n * c->p.n;
+    const int nb = (int)std::min<size_t>((N + PB - 1) / PB, 1024);
+    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X}) HIPCHK(d->ensure(N * sizeof(double)));
+    HIPCHK(c->pz_part.ensure((size_t)nb * 3 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(c->pz_phi.p, phi, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_new.p, b, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_res.p, rho, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_X.p, vxc, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_energy_terms, dim3(nb), dim3(PB), 0, c->st, c->pz_phi.as<double>(), c->pz_new.as<double>(),
+                       c->pz_res.as<double>(), c->pz_X.as<double>(), N, c->p.h, c->pz_part.as<double>());
+    std::vector<double> part((size_t)nb * 3);
+    HIPCHK(hipMemcpyAsync(part.data(), c->pz_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    double t[3] = {0, 0, 0};
+    for (int bb = 0; bb < nb; ++bb)
+        for (int q = 0; q < 3; ++q) t[q] += part[(size_t)bb * 3 + q];
+    double E = t[0] + t[1] + t[2];
+    E = E - eself;
+    E = E + ebs + eent + ecorr;
+    *etot = E / n_atm;
     return SQG_OK;
 }
 

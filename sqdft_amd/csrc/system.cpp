@@ -250,6 +250,8 @@ struct sqdft_system {
     std::vector<int> img_type, img_atom, img_nn;
     std::vector<long> img_off;
     std::vector<double> udv;
+    std::vector<double> rb;        // per type
+    sqdft_poisson_params pois{1e-6, 1000, 0.6, 7, 6};
 };
 
 namespace {
@@ -315,6 +317,7 @@ int build(const sqdft_spec *spec, sqdft_system *S)
             }
         for (int l = 0; l < 4; ++l) ti.denom[l] = P.denom[l];
         if (ti.nproj > I.max_nproj) I.max_nproj = ti.nproj;
+        S->rb.push_back(T.rb);
         for (int a = 0; a < T.natm; ++a) {
             S->atom_xyz.push_back(T.xyz[3 * a]);
             S->atom_xyz.push_back(T.xyz[3 * a + 1]);
@@ -418,6 +421,8 @@ struct InputKeys {
     int n_int[3] = {0, 0, 0};
     int ncell = 1, FD_order = 12, npl = 0, fermi_alg = 1, rand_seed = 0;
     double T = 0, Rcut = 0, lanczos_tol = 0, fermi_tol = 0, perturb = 0;
+    double poisson_tol = 1e-6, beta_aaj = 0.6;
+    int poisson_maxiter = 1000, m_aaj = 7, p_aaj = 6;
     std::string atoms_file;
 };
 
@@ -480,6 +485,11 @@ int sqdft_system_from_files(const char *dir_c, const char *stem, sqdft_system **
         else if (w == "atoms_file") tk.next(K.atoms_file);
         else if (w == "ncell") tk.num(K.ncell);
         else if (w == "perturb") tk.num(K.perturb);
+        else if (w == "poisson_tol") tk.num(K.poisson_tol);
+        else if (w == "poisson_maxiter") tk.num(K.poisson_maxiter);
+        else if (w == "beta_aaj") tk.num(K.beta_aaj);
+        else if (w == "m_aaj") tk.num(K.m_aaj);
+        else if (w == "p_aaj") tk.num(K.p_aaj);
         else if (w == "rand_seed") tk.num(K.rand_seed);
         else if (w == "T") tk.num(K.T);
         else if (w == "npl") tk.num(K.npl);
@@ -549,6 +559,7 @@ int sqdft_system_from_files(const char *dir_c, const char *stem, sqdft_system **
         ts[t].psd_path = paths[t].c_str();
         ts[t].natm = types[t].natm * nc * nc * nc;
         ts[t].xyz = xyz[t].data();
+        ts[t].rb = types[t].rb;
     }
     sqdft_spec sp{};
     sp.n = K.n_int[0] * nc;
@@ -563,7 +574,9 @@ int sqdft_system_from_files(const char *dir_c, const char *stem, sqdft_system **
     sp.fermi_alg = K.fermi_alg;
     sp.n_typ = n_typ;
     sp.types = ts.data();
-    return sqdft_system_create(&sp, out);
+    const int st = sqdft_system_create(&sp, out);
+    if (st == SQDFT_OK) (*out)->pois = {K.poisson_tol, K.poisson_maxiter, K.beta_aaj, K.m_aaj, K.p_aaj};
+    return st;
 }
 
 void sqdft_system_destroy(sqdft_system *S) { delete S; }
@@ -642,6 +655,139 @@ int sqdft_spline_eval(const double *X, const double *Y, const double *YD, int le
     s.D.assign(YD, YD + len);
     for (int i = 0; i < nx; ++i)
         if (!s.eval(x[i], &y[i])) return SQDFT_E_SPLINE;
+    return SQDFT_OK;
+}
+
+int sqdft_system_poisson_params(const sqdft_system *S, sqdft_poisson_params *pp)
+{
+    if (!S || !pp) return SQDFT_E_ARG;
+    *pp = S->pois;
+    return SQDFT_OK;
+}
+
+// ChargeDensity (src/initialize.cpp:471-784) over the rb images of
+// Replica_atoms (src/initialize.cpp:21-150), one domain covering the whole
+// periodic cell.  Per image whose rb-box meets [0, n-1]^3: Vloc_J and the
+// smooth reference potential Vtilde_J on the intersection widened by FDn,
+// b_J = -(1/4 pi) Lap_FD Vloc_J on the intersection (b~ likewise), Vc +=
+// Vtilde_J - Vloc_J, Eself += 1/2 b_J Vloc_J h^3, and rho_at from the
+// atomic density table on the intersection.  The products and sums follow
+// the reference's order term by term so that the printed energy correction
+// is reproduced to its last digit.
+int sqdft_system_pseudocharge(const sqdft_system *S, double *b_out, double *rho_out, double *ncharge,
+                              double *eself, double *ecorr)
+{
+    if (!S) return SQDFT_E_ARG;
+    const sqdft_info &I = S->info;
+    const int n = I.n, F = I.FDn;
+    const double h = I.h, L = I.L;
+    for (int t = 0; t < I.n_typ; ++t)
+        if (!(S->rb[t] > 0)) return SQDFT_E_ARG;
+    const size_t N3 = (size_t)n * n * n;
+    std::vector<double> b(N3, 0.0), bt(N3, 0.0), Vc(N3, 0.0), rho_at(N3, 0.0);
+    const double rc_t = 0.5;   // rc_tilda
+    const double k4pi = -1 / (4 * M_PI);
+    double Eself = 0, Ncharge = 0, Nel = 0, Eself_t = 0, Ncharge_t = 0, Vcon = 0;
+    size_t ia = 0;   // atom index in type order
+    for (int t = 0; t < I.n_typ; ++t) {
+        const PsdTable &P = S->psd[t];
+        Spline sUU;
+        sUU.build(P.r.data(), P.uu.data(), P.size);
+        const double rb = S->rb[t];
+        const int Rb = (int)std::round(rb / h);
+        const double Z = (double)P.Z;
+        for (int a = 0; a < S->tinfo[t].natm; ++a, ++ia) {
+            const double x0 = S->atom_xyz[3 * ia], y0 = S->atom_xyz[3 * ia + 1], z0 = S->atom_xyz[3 * ia + 2];
+            const int px = (int)std::floor(((L + rb - x0) / L) + kTinyR);
+            const int py = (int)std::floor(((L + rb - y0) / L) + kTinyR);
+            const int pz = (int)std::floor(((L + rb - z0) / L) + kTinyR);
+            const int mx = (int)std::ceil(((-rb - x0) / L) - kTinyR);
+            const int my = (int)std::ceil(((-rb - y0) / L) - kTinyR);
+            const int mz = (int)std::ceil(((-rb - z0) / L) - kTinyR);
+            for (int rr = mz; rr <= pz; ++rr)
+                for (int qq = my; qq <= py; ++qq)
+                    for (int pp = mx; pp <= px; ++pp) {
+                        const double Ax = x0 + pp * L, Ay = y0 + qq * L, Az = z0 + rr * L;
+                        const int inn = nearest_node(Ax, h), jnn = nearest_node(Ay, h), knn = nearest_node(Az, h);
+                        // intersection of the rb-box with [0, n-1]^3
+                        const int xs = std::max(inn - Rb, 0), xe = std::min(inn + Rb, n - 1);
+                        const int ys = std::max(jnn - Rb, 0), ye = std::min(jnn + Rb, n - 1);
+                        const int zs = std::max(knn - Rb, 0), ze = std::min(knn + Rb, n - 1);
+                        if (xs > xe || ys > ye || zs > ze) continue;
+                        const int nx = xe - xs + 1 + 2 * F, ny = ye - ys + 1 + 2 * F, nz = ze - zs + 1 + 2 * F;
+                        const int i0 = xs - F, j0 = ys - F, k0 = zs - F;
+                        std::vector<double> V((size_t)nx * ny * nz), Vt((size_t)nx * ny * nz);
+                        auto at = [&](int k, int j, int i) { return ((size_t)k * ny + j) * nx + i; };
+                        for (int k = 0; k < nz; ++k)
+                            for (int j = 0; j < ny; ++j)
+                                for (int i = 0; i < nx; ++i) {
+                                    const double dx = (i0 + i) * h - Ax, dy = (j0 + j) * h - Ay, dz = (k0 + k) * h - Az;
+                                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                                    double v = 0;
+                                    if (!P.sVloc.eval(r, &v)) return SQDFT_E_SPLINE;
+                                    if (std::fabs(r - 0.0) < kTinyR) v = P.rVloc[1] / P.r[1];
+                                    else v = v / r;
+                                    V[at(k, j, i)] = v;
+                                    if (i >= F && i < nx - F && j >= F && j < ny - F && k >= F && k < nz - F) {
+                                        double u = 0;
+                                        if (!sUU.eval(r, &u)) return SQDFT_E_SPLINE;
+                                        rho_at[((size_t)(k + k0) * n + (j + j0)) * n + (i + i0)] += u;
+                                        Nel += u * h * h * h;
+                                    }
+                                    if (r < rc_t)
+                                        Vt[at(k, j, i)] = -Z * (9 * std::pow(r, 7) - 30 * rc_t * std::pow(r, 6) +
+                                                                28 * rc_t * rc_t * std::pow(r, 5) -
+                                                                14 * std::pow(rc_t, 5) * std::pow(r, 2) + 12 * std::pow(rc_t, 7)) /
+                                                          (5 * std::pow(rc_t, 8));
+                                    else
+                                        Vt[at(k, j, i)] = -Z / r;
+                                }
+                        for (int k = zs; k <= ze; ++k)
+                            for (int j = ys; j <= ye; ++j)
+                                for (int i = xs; i <= xe; ++i) {
+                                    const int lk = k - k0, lj = j - j0, li = i - i0;
+                                    double bv = V[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
+                                    double btv = Vt[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
+                                    for (int s = 1; s <= F; ++s) {
+                                        bv += (V[at(lk, lj, li - s)] + V[at(lk, lj, li + s)] + V[at(lk, lj - s, li)] +
+                                               V[at(lk, lj + s, li)] + V[at(lk - s, lj, li)] + V[at(lk + s, lj, li)]) *
+                                              I.coeff_lap[s] * k4pi;
+                                        btv += (Vt[at(lk, lj, li - s)] + Vt[at(lk, lj, li + s)] + Vt[at(lk, lj - s, li)] +
+                                                Vt[at(lk, lj + s, li)] + Vt[at(lk - s, lj, li)] + Vt[at(lk + s, lj, li)]) *
+                                               I.coeff_lap[s] * k4pi;
+                                    }
+                                    const size_t g = ((size_t)k * n + j) * n + i;
+                                    b[g] += bv;
+                                    Eself += 0.5 * bv * V[at(lk, lj, li)] * h * h * h;
+                                    Ncharge += bv * h * h * h;
+                                    bt[g] += btv;
+                                    Eself_t += 0.5 * btv * Vt[at(lk, lj, li)] * h * h * h;
+                                    Ncharge_t += btv * h * h * h;
+                                    Vc[g] += Vt[at(lk, lj, li)] - V[at(lk, lj, li)];
+                                }
+                        if (xs <= 0 && xe >= 0 && ys <= 0 && ye >= 0 && zs <= 0 && ze >= 0)
+                            Vcon += Vt[at(0 - k0, 0 - j0, 0 - i0)] - V[at(0 - k0, 0 - j0, 0 - i0)];
+                    }
+        }
+    }
+    // rho_guess = rho_at scaled to -Ncharge electrons; b~ scaled to Ncharge
+    double Nel2 = 0;
+    for (size_t g = 0; g < N3; ++g) Nel2 += rho_at[g] * h * h * h;
+    (void)Nel;
+    std::vector<double> rho(N3);
+    for (size_t g = 0; g < N3; ++g) rho[g] = std::fabs(Ncharge / Nel2) * rho_at[g];
+    for (size_t g = 0; g < N3; ++g) bt[g] = std::fabs(Ncharge / Ncharge_t) * bt[g];
+    const double V000 = Vc[0];
+    double Ec = 0;
+    for (size_t g = 0; g < N3; ++g) {
+        Vc[g] += -V000 + Vcon;
+        Ec += 0.5 * (bt[g] + b[g]) * Vc[g] * std::pow(h, 3);
+    }
+    if (b_out) std::memcpy(b_out, b.data(), N3 * sizeof(double));
+    if (rho_out) std::memcpy(rho_out, rho.data(), N3 * sizeof(double));
+    if (ncharge) *ncharge = Ncharge;
+    if (eself) *eself = Eself;
+    if (ecorr) *ecorr = Ec + Eself - Eself_t;
     return SQDFT_OK;
 }
 

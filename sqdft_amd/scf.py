@@ -1,0 +1,99 @@
+"""The SCF step around the SQ call, on the device in FP64.
+
+Mirrors the reference's SQDFT_forces / SCF_iteration sequence
+(src/scf.cpp:337-375, :87-200): the pseudocharge b and the starting
+density (ChargeDensity, src/initialize.cpp:471-784, host), the AAJ Poisson
+solve for phi (PoissonSolver_AAJ, src/poisson.cpp:19-325), the LDA
+potential (ExchangeCorrelationPotential, src/scf.cpp:19-64), Veff = phi + Vxc
+(EvaluateEffectivePotential, src/scf.cpp:66-82), then per SCF iteration the
+SQ density, the total energy (EvaluateTotalEnergy, src/energy.cpp:19-87),
+the next Poisson solve from the previous phi and the Veff residual the
+reference prints as SCF_err.  Mixing (src/scf.cpp:200-292) is not part of
+this module: ``iteration`` returns the unmixed Veff_new.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .sq import SpectralQuadrature
+from .system import System
+
+
+class SCF:
+    """Owns a SpectralQuadrature context and the SCF state of one geometry."""
+
+    def __init__(self, system: System, precision: str = "fp64", **sq_kw):
+        self.S = system
+        self.sq = SpectralQuadrature(system, precision=precision, **sq_kw)
+        self._lib = self.sq._lib
+        pc = system.pseudocharge()
+        self.b, self.rho = pc["b"], pc["rho"]
+        self.ncharge, self.eself, self.ecorr = pc["ncharge"], pc["eself"], pc["ecorr"]
+        self.pp = system.poisson_params()
+        self.n3 = system.n ** 3
+        self.phi = None
+        self.vxc = None
+        self.veff = None
+        self.scf_iter = 0
+        self.poisson_log = []   # (iterations, relative residual) of every solve
+
+    def close(self):
+        self.sq.close()
+
+    # ------------------------------------------------------------ kernels
+    def PoissonSolver_AAJ(self, rhs, phi_guess) -> np.ndarray:
+        r = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        g = np.ascontiguousarray(phi_guess, dtype=np.float64).reshape(-1)
+        phi = np.zeros(self.n3)
+        it, rr = C.c_int(), C.c_double()
+        pp = self.pp
+        L.check("sqg_poisson_aaj", self._lib.sqg_poisson_aaj(
+            self.sq._ctx, L.dptr(r), L.dptr(g), pp["tol"], pp["maxiter"], pp["beta"], pp["m"], pp["p"],
+            L.dptr(phi), C.byref(it), C.byref(rr)))
+        self.poisson_log.append((it.value, rr.value))
+        return phi
+
+    def ExchangeCorrelationPotential(self, rho) -> np.ndarray:
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        v = np.zeros(self.n3)
+        L.check("sqg_vxc_lda", self._lib.sqg_vxc_lda(self.sq._ctx, L.dptr(r), L.dptr(v)))
+        return v
+
+    def EvaluateTotalEnergy(self, rho_new, ebs, eent) -> float:
+        """Per atom, with the phi and Vxc of the Veff the SQ call used."""
+        e = C.c_double()
+        arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (self.phi, self.b, rho_new, self.vxc)]
+        L.check("sqg_total_energy", self._lib.sqg_total_energy(
+            self.sq._ctx, *[L.dptr(a) for a in arrs], ebs, eent, self.eself, self.ecorr, self.S.n_atm, C.byref(e)))
+        return e.value
+
+    # ------------------------------------------------------------ driver
+    def initial_potential(self) -> np.ndarray:
+        """phi from rho_at + b with phi_guess = 1, Vxc(rho_at), Veff (src/scf.cpp:353-372)."""
+        self.phi = self.PoissonSolver_AAJ(self.b + self.rho, np.ones(self.n3))
+        self.vxc = self.ExchangeCorrelationPotential(self.rho)
+        self.veff = self.phi + self.vxc
+        return self.veff
+
+    def iteration(self) -> dict:
+        """One SCF iteration on the current Veff (src/scf.cpp:145-200, without mixing)."""
+        if self.veff is None:
+            self.initial_potential()
+        self.scf_iter += 1
+        t0 = time.perf_counter()
+        out = self.sq.density(self.veff, scf_iter=self.scf_iter, ncharge=self.ncharge)
+        t1 = time.perf_counter()
+        etot = self.EvaluateTotalEnergy(out["rho"], out["ebs"], out["eent"])
+        phi = self.PoissonSolver_AAJ(self.b + out["rho"], self.phi)
+        vxc = self.ExchangeCorrelationPotential(out["rho"])
+        veff_new = phi + vxc
+        err = float(np.sqrt(np.sum((veff_new - self.veff) ** 2)) / np.sqrt(np.sum(veff_new ** 2)))
+        t2 = time.perf_counter()
+        self.phi, self.vxc, self.rho = phi, vxc, out["rho"]
+        return {"lambda_f": out["lambda_f"], "etot": etot, "scf_err": err, "rho": out["rho"], "veff_new": veff_new,
+                "ebs": out["ebs"], "eent": out["eent"], "stats": out["stats"],
+                "poisson": self.poisson_log[-1], "t_sq": t1 - t0, "t_rest": t2 - t1}

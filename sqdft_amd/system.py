@@ -25,6 +25,7 @@ class AtomType:
     lloc: int
     psd_path: str
     xyz: np.ndarray          # (natm, 3) Cartesian, bohr
+    rb: float = 0.0          # pseudocharge radius (bohr); 0: no pseudocharge
 
 
 class System:
@@ -54,6 +55,7 @@ class System:
             ts[i].psd_path = str(t.psd_path).encode()
             ts[i].natm = xyz.shape[0]
             ts[i].xyz = L.dptr(xyz)
+            ts[i].rb = float(t.rb)
         spec = L.Spec(n=n, L=L_box, h=h, fd_order=fd_order, Rcut=Rcut, npl=npl, T=T,
                       lanczos_tol=lanczos_tol, fermi_tol=fermi_tol, fermi_alg=fermi_alg,
                       n_typ=len(types), types=ts)
@@ -135,6 +137,23 @@ class System:
         u = np.zeros(max(self._info.udv_len, 1))
         L.check("sqdft_system_udv", self._lib.sqdft_system_udv(self._h, L.dptr(u)))
         return u[: self._info.udv_len]
+
+    def pseudocharge(self) -> dict:
+        """Pseudocharge b, starting density rho (rho_at scaled to -Ncharge
+        electrons), Ncharge, Eself and the energy correction Ecorr
+        (ChargeDensity, src/initialize.cpp:471-784); arrays are n^3, x fastest."""
+        n3 = self.n ** 3
+        b = np.zeros(n3); rho = np.zeros(n3)
+        nc = C.c_double(); es = C.c_double(); ec = C.c_double()
+        L.check("sqdft_system_pseudocharge", self._lib.sqdft_system_pseudocharge(
+            self._h, L.dptr(b), L.dptr(rho), C.byref(nc), C.byref(es), C.byref(ec)))
+        return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value}
+
+    def poisson_params(self) -> dict:
+        """AAJ parameters of the input (poisson_tol, poisson_maxiter, beta_aaj, m_aaj, p_aaj)."""
+        pp = L.PoissonParams()
+        L.check("sqdft_system_poisson_params", self._lib.sqdft_system_poisson_params(self._h, C.byref(pp)))
+        return {k: getattr(pp, k) for k, _ in pp._fields_}
 
     def total_valence(self) -> int:
         return int(sum(self.type_info(t)["Z"] * self.type_info(t)["natm"] for t in range(self._info.n_typ)))

@@ -1,0 +1,78 @@
+"""The SCF step on the GPU (AAJ Poisson, LDA potential, total energy) through
+the C-ABI: against the FP64 oracle on the same inputs, and the whole first
+SCF iteration of the reference's Al test against its log
+(tests/Al/results_Al:78-84)."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+from sqdft_amd.scf import SCF
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def al():
+    S, Rcut, T = H.test_al()
+    return S, Rcut, T
+
+
+def test_poisson_and_vxc_match_oracle(al):
+    S, _, _ = al
+    say("scf: poisson and vxc vs oracle")
+    scf = SCF(S)
+    info = S.info
+    n, F = info["n"], info["FDn"]
+    lap = info["coeff_lap"][: F + 1]
+    pp = S.poisson_params()
+    rhs = scf.b + scf.rho
+    g = scf.PoissonSolver_AAJ(rhs, np.ones(n ** 3))
+    git, grr = scf.poisson_log[-1]
+    o, oit, orr = O.poisson_aaj(n, F, lap, rhs, np.ones(n ** 3), pp["tol"], pp["maxiter"], pp["beta"], pp["m"], pp["p"])
+    # same iterations; the Anderson steps solve a 7x7 Gram system whose
+    # condition number amplifies the different summation order of the
+    # device reductions, so the iterates agree to ~1e-9, not to 1e-15
+    dphi = np.max(np.abs(g - o)) / np.max(np.abs(o))
+    say("poisson: its", git, oit, "relres", grr, orr, "max rel phi diff", dphi)
+    assert git == oit
+    assert abs(grr - orr) <= 1e-6 * orr
+    assert dphi <= 1e-9
+    v = scf.ExchangeCorrelationPotential(scf.rho)
+    ov = O.vxc_lda(scf.rho)
+    assert np.max(np.abs(v - ov)) <= 1e-13 * np.max(np.abs(ov))
+    scf.close()
+
+
+def test_first_scf_iteration_on_gpu_matches_log(al):
+    S, _, _ = al
+    say("scf: first iteration on the GPU path")
+    scf = SCF(S)
+    scf.initial_potential()
+    assert (scf.poisson_log[0][0], "%g" % scf.poisson_log[0][1]) == (46, "9.71123e-07")
+    r = scf.iteration()
+    st = r["stats"]
+    assert st["lanczos_iters_node0"] == 35 and st["fermi_iters"] == 15
+    assert "%.12f" % r["lambda_f"] == "0.236781162275"
+    assert "%.12f" % r["etot"] == "-2.465517914262"
+    assert (r["poisson"][0], "%g" % r["poisson"][1]) == (37, "1.51603e-07")
+    assert "%g" % r["scf_err"] == "0.244939"
+    scf.close()
+
+
+def test_mixed_precision_first_iteration(al):
+    # FP32 storage in the SQ step only; Poisson, Vxc and energy stay FP64
+    S, _, _ = al
+    say("scf: first iteration, mixed")
+    s64, s32 = SCF(S), SCF(S, precision="mixed")
+    a, b = s64.iteration(), s32.iteration()
+    assert abs(a["lambda_f"] - b["lambda_f"]) < 1e-5
+    assert abs(a["etot"] - b["etot"]) < 1e-5
+    s64.close(); s32.close()

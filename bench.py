@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the steps, run one plain device copy of this size (MiB) so that "
                          "counter runs can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -213,6 +214,23 @@ def main():
                         "lambda_f_err": abs(t32["lambda_f"] - t64["lambda_f"]),
                         "rho_rel_err_at_fermi_tol_1e-6": err,
                         "ebs_rel_err": abs(t32["ebs"] - t64["ebs"]) / abs(t64["ebs"])}
+    sq.close()
+    if not args.no_scf:
+        # one SCF iteration on the workload's own pseudocharge: SQ density,
+        # total energy, AAJ Poisson from the previous phi, LDA potential and the
+        # Veff residual (src/scf.cpp:145-200); the second iteration is timed
+        log("bench: scf iterations")
+        from sqdft_amd.scf import SCF
+        scf = SCF(S, precision=args.precision, batch_nodes=args.batch_nodes)
+        scf.initial_potential()
+        it1 = scf.iteration()
+        scf.veff = it1["veff_new"]
+        it2 = scf.iteration()
+        scf.close()
+        out["scf_step"] = {"s": it2["t_sq"] + it2["t_rest"], "sq_s": it2["t_sq"],
+                           "energy_poisson_vxc_s": it2["t_rest"], "poisson_iters": it2["poisson"][0],
+                           "scf_err": it2["scf_err"], "lambda_f": it2["lambda_f"],
+                           "what": "second SCF iteration from the atomic starting density, unmixed Veff"}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(S, veff, cfg["Rcut"], cfg["T"], args.cpu_seconds)

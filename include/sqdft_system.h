@@ -116,16 +116,21 @@ int  sqdft_system_udv(const sqdft_system *S, double *udv);
 int  sqdft_system_pseudocharge(const sqdft_system *S, double *b, double *rho, double *ncharge,
                                double *eself, double *ecorr);
 
-/* Poisson-solver and SCF-mixing parameters (".input" poisson_tol,
- * poisson_maxiter, beta_aaj, m_aaj, p_aaj; defaults 1e-6, 1000, 0.6, 7, 6
- * for systems built from a spec). */
-typedef struct sqdft_poisson_params {
-    double tol;
+/* Poisson-solver and SCF parameters (".input" poisson_tol, poisson_maxiter,
+ * beta_aaj, m_aaj, p_aaj, scf_tol, scf_miniter, scf_maxiter, beta_scf,
+ * m_scf, p_scf; defaults 1e-6, 1000, 0.6, 7, 6, 1e-4, 3, 40, 0.1, 7, 2 for
+ * systems built from a spec). */
+typedef struct sqdft_scf_params {
+    double tol;               /* Poisson (AAJ) */
     int maxiter;
     double beta;
     int m, p;
-} sqdft_poisson_params;
-int  sqdft_system_poisson_params(const sqdft_system *S, sqdft_poisson_params *pp);
+    double scf_tol;           /* SCF loop and Periodic Pulay mixing of Veff */
+    int scf_miniter, scf_maxiter;
+    double beta_scf;
+    int m_scf, p_scf;
+} sqdft_scf_params;
+int  sqdft_system_scf_params(const sqdft_system *S, sqdft_scf_params *pp);
 /* Cubic-spline evaluation with the reference's routine (tests). */
 int  sqdft_spline_eval(const double *X, const double *Y, const double *YD, int len,
                        const double *x, double *y, int nx);

@@ -21,9 +21,10 @@ class TypeSpec(C.Structure):
                 ("psd_path", C.c_char_p), ("natm", C.c_int), ("xyz", c_dbl_p), ("rb", C.c_double)]
 
 
-class PoissonParams(C.Structure):
+class ScfParams(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("beta", C.c_double),
-                ("m", C.c_int), ("p", C.c_int)]
+                ("m", C.c_int), ("p", C.c_int), ("scf_tol", C.c_double), ("scf_miniter", C.c_int),
+                ("scf_maxiter", C.c_int), ("beta_scf", C.c_double), ("m_scf", C.c_int), ("p_scf", C.c_int)]
 
 
 class Spec(C.Structure):
@@ -86,7 +87,7 @@ SIGNATURES = {
     "sqg_total_energy": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_int, c_dbl_p]),
     "sqdft_system_pseudocharge": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
-    "sqdft_system_poisson_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sqdft_system_scf_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sqdft_spline_eval": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, C.c_int]),
     "sqdft_spline_derivs": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int]),
     # sqdft_gpu.h

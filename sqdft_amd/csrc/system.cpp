@@ -251,7 +251,7 @@ struct sqdft_system {
     std::vector<long> img_off;
     std::vector<double> udv;
     std::vector<double> rb;        // per type
-    sqdft_poisson_params pois{1e-6, 1000, 0.6, 7, 6};
+    sqdft_scf_params pois{1e-6, 1000, 0.6, 7, 6, 1e-4, 3, 40, 0.1, 7, 2};
 };
 
 namespace {
@@ -421,8 +421,8 @@ struct InputKeys {
     int n_int[3] = {0, 0, 0};
     int ncell = 1, FD_order = 12, npl = 0, fermi_alg = 1, rand_seed = 0;
     double T = 0, Rcut = 0, lanczos_tol = 0, fermi_tol = 0, perturb = 0;
-    double poisson_tol = 1e-6, beta_aaj = 0.6;
-    int poisson_maxiter = 1000, m_aaj = 7, p_aaj = 6;
+    double poisson_tol = 1e-6, beta_aaj = 0.6, scf_tol = 1e-4, beta_scf = 0.1;
+    int poisson_maxiter = 1000, m_aaj = 7, p_aaj = 6, scf_miniter = 3, scf_maxiter = 40, m_scf = 7, p_scf = 2;
     std::string atoms_file;
 };
 
@@ -490,6 +490,12 @@ int sqdft_system_from_files(const char *dir_c, const char *stem, sqdft_system **
         else if (w == "beta_aaj") tk.num(K.beta_aaj);
         else if (w == "m_aaj") tk.num(K.m_aaj);
         else if (w == "p_aaj") tk.num(K.p_aaj);
+        else if (w == "scf_tol") tk.num(K.scf_tol);
+        else if (w == "scf_miniter") tk.num(K.scf_miniter);
+        else if (w == "scf_maxiter") tk.num(K.scf_maxiter);
+        else if (w == "beta_scf") tk.num(K.beta_scf);
+        else if (w == "m_scf") tk.num(K.m_scf);
+        else if (w == "p_scf") tk.num(K.p_scf);
         else if (w == "rand_seed") tk.num(K.rand_seed);
         else if (w == "T") tk.num(K.T);
         else if (w == "npl") tk.num(K.npl);
@@ -575,7 +581,9 @@ int sqdft_system_from_files(const char *dir_c, const char *stem, sqdft_system **
     sp.n_typ = n_typ;
     sp.types = ts.data();
     const int st = sqdft_system_create(&sp, out);
-    if (st == SQDFT_OK) (*out)->pois = {K.poisson_tol, K.poisson_maxiter, K.beta_aaj, K.m_aaj, K.p_aaj};
+    if (st == SQDFT_OK)
+        (*out)->pois = {K.poisson_tol, K.poisson_maxiter, K.beta_aaj, K.m_aaj,    K.p_aaj,
+                        K.scf_tol,     K.scf_miniter,     K.scf_maxiter, K.beta_scf, K.m_scf, K.p_scf};
     return st;
 }
 
@@ -658,7 +666,7 @@ int sqdft_spline_eval(const double *X, const double *Y, const double *YD, int le
     return SQDFT_OK;
 }
 
-int sqdft_system_poisson_params(const sqdft_system *S, sqdft_poisson_params *pp)
+int sqdft_system_scf_params(const sqdft_system *S, sqdft_scf_params *pp)
 {
     if (!S || !pp) return SQDFT_E_ARG;
     *pp = S->pois;

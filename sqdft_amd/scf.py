@@ -8,8 +8,9 @@ potential (ExchangeCorrelationPotential, src/scf.cpp:19-64), Veff = phi + Vxc
 (EvaluateEffectivePotential, src/scf.cpp:66-82), then per SCF iteration the
 SQ density, the total energy (EvaluateTotalEnergy, src/energy.cpp:19-87),
 the next Poisson solve from the previous phi and the Veff residual the
-reference prints as SCF_err.  Mixing (src/scf.cpp:200-292) is not part of
-this module: ``iteration`` returns the unmixed Veff_new.
+reference prints as SCF_err.  ``iteration`` returns the unmixed Veff_new;
+``run`` adds the reference's mixing (simple mixing with a Periodic Pulay /
+Anderson step every p_scf iterations, src/scf.cpp:200-292) on the host.
 """
 from __future__ import annotations
 
@@ -21,6 +22,19 @@ import numpy as np
 from . import _lib as L
 from .sq import SpectralQuadrature
 from .system import System
+
+
+def anderson_extrapolation(DX: np.ndarray, DF: np.ndarray, res: np.ndarray, beta: float) -> np.ndarray:
+    """Anderson update of AndersonExtrapolation (src/anderson.cpp:20-100):
+    x = pinv(DF DF') DF res (exactly zero singular values dropped, as
+    PseudoInverseTimesVec does), update = sum_j (DX_j + beta DF_j) x_j."""
+    FtF = DF @ DF.T
+    Ftf = DF @ res
+    U, s, Vt = np.linalg.svd(FtF)
+    c = U.T @ Ftf
+    c = np.where(s != 0, c / np.where(s != 0, s, 1.0), 0.0)
+    x = Vt.T @ c
+    return (DX + beta * DF).T @ x
 
 
 class SCF:
@@ -97,3 +111,40 @@ class SCF:
         return {"lambda_f": out["lambda_f"], "etot": etot, "scf_err": err, "rho": out["rho"], "veff_new": veff_new,
                 "ebs": out["ebs"], "eent": out["eent"], "stats": out["stats"],
                 "poisson": self.poisson_log[-1], "t_sq": t1 - t0, "t_rest": t2 - t1}
+
+    def run(self) -> dict:
+        """SCF loop to scf_tol (SCF_iteration, src/scf.cpp:84-330): per
+        iteration the SQ density, energy, Poisson and Vxc, then Veff mixed as
+        Veff_old + beta_scf res with an Anderson step every p_scf iterations
+        over the last m_scf differences."""
+        pp = self.pp
+        m, p, beta = pp["m_scf"], pp["p_scf"], pp["beta_scf"]
+        if self.veff is None:
+            self.initial_potential()
+        N = self.n3
+        DX, DF = np.zeros((m, N)), np.zeros((m, N))
+        Xold, Fold = np.zeros(N), np.zeros(N)
+        veff_old = self.veff.copy()
+        history = []
+        err, count, last = pp["scf_tol"] + 1.0, 1, None
+        while err > pp["scf_tol"] and count <= pp["scf_maxiter"]:
+            last = self.iteration()
+            err = last["scf_err"]
+            res = last["veff_new"] - veff_old
+            vnew = veff_old + beta * res
+            if count > 1:
+                k = (count - 2) % m
+                DX[k] = veff_old - Xold
+                DF[k] = res - Fold
+            Xold, Fold = veff_old.copy(), res.copy()
+            if count % p == 0 and count > 1:
+                vnew = vnew - anderson_extrapolation(DX, DF, res, beta)
+            veff_old = vnew
+            if err > pp["scf_tol"] and count < pp["scf_maxiter"]:
+                self.veff = vnew
+            history.append({"lambda_f": last["lambda_f"], "etot": last["etot"], "scf_err": err,
+                            "poisson": last["poisson"], "fermi_iters": last["stats"]["fermi_iters"],
+                            "t_sq": last["t_sq"], "t_rest": last["t_rest"]})
+            count += 1
+        return {"iterations": count - 1, "converged": err <= pp["scf_tol"], "free_energy": last["etot"],
+                "lambda_f": last["lambda_f"], "history": history}

@@ -150,9 +150,10 @@ class System:
         return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value}
 
     def poisson_params(self) -> dict:
-        """AAJ parameters of the input (poisson_tol, poisson_maxiter, beta_aaj, m_aaj, p_aaj)."""
-        pp = L.PoissonParams()
-        L.check("sqdft_system_poisson_params", self._lib.sqdft_system_poisson_params(self._h, C.byref(pp)))
+        """AAJ and SCF parameters of the input: tol, maxiter, beta, m, p of the
+        Poisson solver; scf_tol, scf_miniter, scf_maxiter, beta_scf, m_scf, p_scf."""
+        pp = L.ScfParams()
+        L.check("sqdft_system_scf_params", self._lib.sqdft_system_scf_params(self._h, C.byref(pp)))
         return {k: getattr(pp, k) for k, _ in pp._fields_}
 
     def total_valence(self) -> int:

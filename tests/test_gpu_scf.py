@@ -76,3 +76,15 @@ def test_mixed_precision_first_iteration(al):
     assert abs(a["lambda_f"] - b["lambda_f"]) < 1e-5
     assert abs(a["etot"] - b["etot"]) < 1e-5
     s64.close(); s32.close()
+
+
+def test_scf_on_gpu_converges_like_log(al):
+    from test_scf_oracle import check_against_al_log
+    S, _, _ = al
+    say("scf: full SCF on the GPU path")
+    scf = SCF(S)
+    out = scf.run()
+    assert out["converged"] and out["iterations"] == 9
+    check_against_al_log(out["history"])
+    assert abs(out["free_energy"] - (-2.453313155422)) < 3e-12
+    scf.close()

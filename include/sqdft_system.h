@@ -76,6 +76,7 @@ typedef struct sqdft_type_info {
     double rc, rz;           /* grid-rounded radii (readfiles.cpp:1121-1124) */
     double denom[4];         /* Kleinman-Bylander denominators per l (carry 1/h^3) */
     int size;                /* radial grid length (incl. r = 0) */
+    double rb;               /* pseudocharge radius (bohr) */
 } sqdft_type_info;
 
 typedef struct sqdft_system sqdft_system;

@@ -93,6 +93,8 @@ def load() -> C.CDLL:
         "sqo_poisson_aaj": (C.c_int, [C.c_int, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_int,
                                       C.c_double, C.c_int, C.c_int, c_dbl_p, c_int_p, c_dbl_p]),
         "sqo_vxc_lda": (None, [C.c_size_t, c_dbl_p, c_dbl_p]),
+        "sqo_charge_density": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+        "sqo_forces_local": (C.c_int, [P, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
         "sqo_total_energy": (C.c_double, [C.c_size_t, C.c_double, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double,
                                           C.c_double, C.c_double, C.c_double, C.c_int]),
     }
@@ -276,6 +278,27 @@ class OracleSystem:
         if st < 0:
             raise RuntimeError(f"sqo_forces_nonlocal: {st}")
         return f.reshape(3, na).T.copy()
+
+    def charge_density(self, rb) -> dict:
+        """ChargeDensity at the first MD step (initialize.cpp:471-784)."""
+        N = self.n ** 3
+        rb = np.ascontiguousarray(rb, dtype=np.float64)
+        b, bt, vc, rho, sc = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(5)
+        st = self._lib.sqo_charge_density(C.byref(self.S), _d(rb), _d(b), _d(bt), _d(vc), _d(rho), _d(sc))
+        if st < 0:
+            raise RuntimeError(f"sqo_charge_density: {st}")
+        return {"b": b, "b_tilda": bt, "vc": vc, "rho": rho, "ncharge": sc[0], "eself": sc[1],
+                "ncharge_tilda": sc[2], "eself_tilda": sc[3], "ecorr": sc[4]}
+
+    def forces_local(self, rb, phi, b, bt, vc):
+        """ForcesLocal (forces.cpp:20-315): (floc, fcorr), each (n_atm, 3)."""
+        na = self.S.n_atm
+        fl, fc = np.zeros(3 * na), np.zeros(3 * na)
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (rb, phi, b, bt, vc)]
+        st = self._lib.sqo_forces_local(C.byref(self.S), *[_d(a) for a in arr], _d(fl), _d(fc))
+        if st < 0:
+            raise RuntimeError(f"sqo_forces_local: {st}")
+        return fl.reshape(3, na).T.copy(), fc.reshape(3, na).T.copy()
 
 
 def cheb_coeff(npl, lfp, bet, func=0) -> np.ndarray:

@@ -1563,3 +1563,225 @@ double sqo_total_energy(size_t N, double h, const double *phi, const double *b, 
     Etot = Etot + Ebs + Eent + Ecorr;
     return Etot / n_atm;
 }
+
+/* ------------------------------------------------------------------ */
+/* Pseudocharge and local forces: Replica_atoms (initialize.cpp:21-200),*/
+/* ChargeDensity (initialize.cpp:471-784), ForcesLocal (forces.cpp:20-315)*/
+/* on one rank: the processor domain is [0, n-1]^3.                      */
+/* ------------------------------------------------------------------ */
+typedef struct { double x, y, z; int lo[3], hi[3]; int type, atom; } rb_img;
+
+/* periodic replicas whose rb-box around the nearest node meets the domain,
+   in the reference's (type, atom, z, y, x) order (initialize.cpp:34-122) */
+static int rb_images(const sqo_sys *S, const double *rb, rb_img **out)
+{
+    int cap = 64, cnt = 0, a = 0;
+    rb_img *v = (rb_img *)malloc(cap * sizeof(rb_img));
+    const double L = S->L, h = S->h;
+    for (int t = 0; t < S->n_typ; t++) {
+        int Rb = (int)round(rb[t] / h);
+        for (int q = 0; q < S->natm[t]; q++, a++) {
+            double c[3] = {S->ax[a], S->ay[a], S->az[a]};
+            int pm[3], mm[3];
+            for (int d = 0; d < 3; d++) {
+                pm[d] = (int)floor(((L + rb[t] - c[d]) / L) + TEMP_TOL);
+                mm[d] = (int)ceil(((-rb[t] - c[d]) / L) - TEMP_TOL);
+            }
+            for (int rr = mm[2]; rr <= pm[2]; rr++)
+                for (int qq = mm[1]; qq <= pm[1]; qq++)
+                    for (int pp = mm[0]; pp <= pm[0]; pp++) {
+                        rb_img I;
+                        I.x = c[0] + pp * L; I.y = c[1] + qq * L; I.z = c[2] + rr * L;
+                        double p3[3] = {I.x, I.y, I.z};
+                        int ok = 1;
+                        for (int d = 0; d < 3; d++) {
+                            int nn = nn1(p3[d], h);
+                            I.lo[d] = imax(nn - Rb, 0);
+                            I.hi[d] = imin(nn + Rb, S->n - 1);
+                            if (I.lo[d] > I.hi[d]) ok = 0;
+                        }
+                        if (!ok) continue;
+                        I.type = t; I.atom = a;
+                        if (cnt == cap) { cap *= 2; v = (rb_img *)realloc(v, cap * sizeof(rb_img)); }
+                        v[cnt++] = I;
+                    }
+        }
+    }
+    *out = v;
+    return cnt;
+}
+
+/* V_J and the smooth reference potential V~_J (rc~ = 0.5) on the image's
+   intersection box widened by mg nodes (initialize.cpp:565-595) */
+static int vloc_box(const sqo_sys *S, const rb_img *I, int mg, const double *DVloc, double *V, double *Vt)
+{
+    const sqo_psd *P = &S->psd[I->type];
+    const double h = S->h, rc = 0.5, Z = S->Z[I->type];
+    int nx = I->hi[0] - I->lo[0] + 1 + 2 * mg, ny = I->hi[1] - I->lo[1] + 1 + 2 * mg;
+    int nz = I->hi[2] - I->lo[2] + 1 + 2 * mg;
+    int i0 = I->lo[0] - mg, j0 = I->lo[1] - mg, k0 = I->lo[2] - mg;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double dx = (i0 + i) * h - I->x, dy = (j0 + j) * h - I->y, dz = (k0 + k) * h - I->z;
+                double r = sqrt(dx * dx + dy * dy + dz * dz), v;
+                if (sqo_spline_eval(P->r, P->rVloc, P->size, &r, &v, 1, DVloc)) return SQO_ERR_SPLINE_RANGE;
+                long g = ((long)k * ny + j) * nx + i;
+                V[g] = fabs(r - 0.0) < TEMP_TOL ? P->rVloc[1] / P->r[1] : v / r;
+                Vt[g] = r < rc ? -Z * (9 * pow(r, 7) - 30 * rc * pow(r, 6) + 28 * rc * rc * pow(r, 5) -
+                                       14 * pow(rc, 5) * pow(r, 2) + 12 * pow(rc, 7)) / (5 * pow(rc, 8))
+                               : -Z / r;
+            }
+    return SQO_OK;
+}
+
+/* -(1/4pi) Lap_FD of a box array at interior point (i,j,k) */
+static double neg_lap_4pi(const sqo_sys *S, const double *V, int nx, int ny, int i, int j, int k)
+{
+    const double q = -1 / (4 * M_PI);
+#define AT(kk, jj, ii) V[((long)(kk) * ny + (jj)) * nx + (ii)]
+    double s = AT(k, j, i) * 3 * S->coeff_lap[0] * q;
+    for (int a = 1; a <= S->FDn; a++)
+        s += (AT(k, j, i - a) + AT(k, j, i + a) + AT(k, j - a, i) + AT(k, j + a, i) + AT(k - a, j, i) +
+              AT(k + a, j, i)) * S->coeff_lap[a] * q;
+#undef AT
+    return s;
+}
+
+int sqo_charge_density(const sqo_sys *S, const double *rb, double *b, double *bt, double *vc,
+                       double *rho, double *scal)
+{
+    const int n = S->n, F = S->FDn;
+    const double h = S->h, h3 = h * h * h;
+    const long N3 = (long)n * n * n;
+    double *rho_at = (double *)calloc(N3, sizeof(double));
+    for (long g = 0; g < N3; g++) b[g] = bt[g] = vc[g] = 0.0;
+    double Eself = 0, Ncharge = 0, Eself_t = 0, Ncharge_t = 0, Vcon = 0;
+    rb_img *im;
+    int nim = rb_images(S, rb, &im), st = SQO_OK;
+    double *DV = NULL, *Duu = NULL;
+    int cur_t = -1;
+    for (int m = 0; m < nim && st == SQO_OK; m++) {
+        const rb_img *I = &im[m];
+        const sqo_psd *P = &S->psd[I->type];
+        if (I->type != cur_t) {   /* initialize.cpp:508-512 */
+            free(DV); free(Duu);
+            DV = (double *)malloc(P->size * sizeof(double));
+            Duu = (double *)malloc(P->size * sizeof(double));
+            sqo_spline_yd(P->r, P->rVloc, DV, P->size);
+            sqo_spline_yd(P->r, P->uu, Duu, P->size);
+            cur_t = I->type;
+        }
+        int nx = I->hi[0] - I->lo[0] + 1 + 2 * F, ny = I->hi[1] - I->lo[1] + 1 + 2 * F;
+        int nz = I->hi[2] - I->lo[2] + 1 + 2 * F;
+        int i0 = I->lo[0] - F, j0 = I->lo[1] - F, k0 = I->lo[2] - F;
+        double *V = (double *)malloc((size_t)nx * ny * nz * sizeof(double));
+        double *Vt = (double *)malloc((size_t)nx * ny * nz * sizeof(double));
+        st = vloc_box(S, I, F, DV, V, Vt);
+        /* atomic valence density on the intersection (initialize.cpp:582-587) */
+        for (int k = F; k < nz - F && st == SQO_OK; k++)
+            for (int j = F; j < ny - F; j++)
+                for (int i = F; i < nx - F; i++) {
+                    double dx = (i0 + i) * h - I->x, dy = (j0 + j) * h - I->y, dz = (k0 + k) * h - I->z;
+                    double r = sqrt(dx * dx + dy * dy + dz * dz), u;
+                    if (sqo_spline_eval(P->r, P->uu, P->size, &r, &u, 1, Duu)) { st = SQO_ERR_SPLINE_RANGE; break; }
+                    rho_at[((long)(k + k0) * n + (j + j0)) * n + (i + i0)] += u;
+                }
+        /* b_J, b~_J and their sums (initialize.cpp:599-630) */
+        for (int k = I->lo[2]; k <= I->hi[2] && st == SQO_OK; k++)
+            for (int j = I->lo[1]; j <= I->hi[1]; j++)
+                for (int i = I->lo[0]; i <= I->hi[0]; i++) {
+                    int li = i - i0, lj = j - j0, lk = k - k0;
+                    long l = ((long)lk * ny + lj) * nx + li, g = ((long)k * n + j) * n + i;
+                    double bj = neg_lap_4pi(S, V, nx, ny, li, lj, lk);
+                    double btj = neg_lap_4pi(S, Vt, nx, ny, li, lj, lk);
+                    b[g] += bj;
+                    Eself += 0.5 * bj * V[l] * h3;
+                    Ncharge += bj * h3;
+                    bt[g] += btj;
+                    Eself_t += 0.5 * btj * Vt[l] * h3;
+                    Ncharge_t += btj * h3;
+                    vc[g] += Vt[l] - V[l];
+                }
+        if (I->lo[0] <= 0 && I->lo[1] <= 0 && I->lo[2] <= 0)   /* node (0,0,0), initialize.cpp:633-636 */
+            Vcon += Vt[((long)(0 - k0) * ny + (0 - j0)) * nx + (0 - i0)] - V[((long)(0 - k0) * ny + (0 - j0)) * nx + (0 - i0)];
+        free(V); free(Vt);
+    }
+    free(DV); free(Duu); free(im);
+    if (st != SQO_OK) { free(rho_at); return st; }
+    /* rho = rho_at scaled to Ncharge electrons, b~ scaled to Ncharge, Vc
+       shifted, Ecorr (initialize.cpp:661-779 at the first MD step) */
+    double Nel = 0;
+    for (long g = 0; g < N3; g++) Nel += rho_at[g] * h3;
+    for (long g = 0; g < N3; g++) rho[g] = fabs(Ncharge / Nel) * rho_at[g];
+    for (long g = 0; g < N3; g++) bt[g] = fabs(Ncharge / Ncharge_t) * bt[g];
+    double V000 = vc[0], Ec = 0;
+    for (long g = 0; g < N3; g++) {
+        vc[g] += -V000 + Vcon;
+        Ec += 0.5 * (bt[g] + b[g]) * vc[g] * pow(h, 3);
+    }
+    free(rho_at);
+    scal[0] = Ncharge; scal[1] = Eself; scal[2] = Ncharge_t; scal[3] = Eself_t;
+    scal[4] = Ec + Eself - Eself_t;
+    return SQO_OK;
+}
+
+int sqo_forces_local(const sqo_sys *S, const double *rb, const double *phi, const double *b,
+                     const double *bt, const double *vc, double *floc, double *fcorr)
+{
+    const int n = S->n, F = S->FDn, na = S->n_atm;
+    const double h3 = pow(S->h, 3);
+    for (int a = 0; a < 3 * na; a++) floc[a] = fcorr[a] = 0.0;
+    rb_img *im;
+    int nim = rb_images(S, rb, &im), st = SQO_OK;
+    double *DV = NULL;
+    int cur_t = -1;
+    for (int m = 0; m < nim && st == SQO_OK; m++) {
+        const rb_img *I = &im[m];
+        const sqo_psd *P = &S->psd[I->type];
+        if (I->type != cur_t) {   /* forces.cpp:51-52 */
+            free(DV);
+            DV = (double *)malloc(P->size * sizeof(double));
+            sqo_spline_yd(P->r, P->rVloc, DV, P->size);
+            cur_t = I->type;
+        }
+        /* V on the intersection widened by 2F, b_J on it widened by F */
+        int nx = I->hi[0] - I->lo[0] + 1 + 4 * F, ny = I->hi[1] - I->lo[1] + 1 + 4 * F;
+        int nz = I->hi[2] - I->lo[2] + 1 + 4 * F;
+        int i0 = I->lo[0] - 2 * F, j0 = I->lo[1] - 2 * F, k0 = I->lo[2] - 2 * F;
+        size_t NB = (size_t)nx * ny * nz;
+        double *V = (double *)malloc(NB * sizeof(double)), *Vt = (double *)malloc(NB * sizeof(double));
+        double *bJ = (double *)calloc(NB, sizeof(double)), *bJt = (double *)calloc(NB, sizeof(double));
+        st = vloc_box(S, I, 2 * F, DV, V, Vt);
+        for (int k = F; k < nz - F && st == SQO_OK; k++)   /* forces.cpp:158-177 */
+            for (int j = F; j < ny - F; j++)
+                for (int i = F; i < nx - F; i++) {
+                    long l = ((long)k * ny + j) * nx + i;
+                    bJ[l] = neg_lap_4pi(S, V, nx, ny, i, j, k);
+                    bJt[l] = neg_lap_4pi(S, Vt, nx, ny, i, j, k);
+                }
+        const long sx = 1, sy = nx, sz = (long)nx * ny;
+        const long step[3] = {sx, sy, sz};
+        double *fl[3] = {floc, floc + na, floc + 2 * na}, *fc[3] = {fcorr, fcorr + na, fcorr + 2 * na};
+        for (int k = 2 * F; k < nz - 2 * F && st == SQO_OK; k++)   /* forces.cpp:183-276 */
+            for (int j = 2 * F; j < ny - 2 * F; j++)
+                for (int i = 2 * F; i < nx - 2 * F; i++) {
+                    long l = ((long)k * ny + j) * nx + i;
+                    long g = ((long)(k + k0) * n + (j + j0)) * n + (i + i0);
+                    for (int d = 0; d < 3; d++) {
+                        long s = step[d];
+                        double Db = 0, Dbt = 0, DVv = 0, DVt = 0;
+                        for (int a = 1; a <= F; a++) Db += (-bJ[l - a * s] + bJ[l + a * s]) * S->coeff_grad[a];
+                        fl[d][I->atom] += Db * (phi[g] - V[l]) * h3;
+                        for (int a = 1; a <= F; a++) Dbt += (-bJt[l - a * s] + bJt[l + a * s]) * S->coeff_grad[a];
+                        for (int a = 1; a <= F; a++) DVv += (-V[l - a * s] + V[l + a * s]) * S->coeff_grad[a];
+                        for (int a = 1; a <= F; a++) DVt += (-Vt[l - a * s] + Vt[l + a * s]) * S->coeff_grad[a];
+                        fc[d][I->atom] += 0.5 * (Dbt * (vc[g] - Vt[l]) + Db * (vc[g] + V[l]) +
+                                                 (DVt - DVv) * (bt[g] + b[g]) + bJ[l] * DVv - bJt[l] * DVt) * h3;
+                    }
+                }
+        free(V); free(Vt); free(bJ); free(bJt);
+    }
+    free(DV); free(im);
+    return st;
+}

@@ -188,6 +188,17 @@ void sqo_vxc_lda(size_t N, const double *rho, double *vxc);
 /* Total energy per atom of EvaluateTotalEnergy (energy.cpp:19-87). */
 double sqo_total_energy(size_t N, double h, const double *phi, const double *b, const double *rho,
                         const double *vxc, double Ebs, double Eent, double Eself, double Ecorr, int n_atm);
+
+/* Pseudocharge of ChargeDensity (initialize.cpp:471-784, first MD step) over
+ * the rb-replicas of Replica_atoms (initialize.cpp:21-200), one rank: b, b~
+ * (scaled to Ncharge), Vc (shifted), rho = rho_at scaled to Ncharge electrons;
+ * scal = {Ncharge, Eself, Ncharge~, Eself~, Ecorr}.  rb per type (bohr). */
+int sqo_charge_density(const sqo_sys *S, const double *rb, double *b, double *bt, double *vc,
+                       double *rho, double *scal);
+/* ForcesLocal (forces.cpp:20-315): floc = sum_J grad(b_J) (phi - V_J) h^3 and
+ * the correction fcorr of the same loop; both laid out fx[n_atm], fy, fz. */
+int sqo_forces_local(const sqo_sys *S, const double *rb, const double *phi, const double *b,
+                     const double *bt, const double *vc, double *floc, double *fcorr);
 #ifdef __cplusplus
 }
 #endif

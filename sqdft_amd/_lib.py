@@ -46,7 +46,7 @@ class Info(C.Structure):
 class TypeInfo(C.Structure):
     _fields_ = [("Z", C.c_int), ("lloc", C.c_int), ("lmax", C.c_int), ("natm", C.c_int),
                 ("nlocc", C.c_int), ("nproj", C.c_int), ("rc", C.c_double), ("rz", C.c_double),
-                ("denom", C.c_double * 4), ("size", C.c_int)]
+                ("denom", C.c_double * 4), ("size", C.c_int), ("rb", C.c_double)]
 
 
 class Params(C.Structure):

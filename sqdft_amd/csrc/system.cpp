@@ -317,6 +317,7 @@ int build(const sqdft_spec *spec, sqdft_system *S)
             }
         for (int l = 0; l < 4; ++l) ti.denom[l] = P.denom[l];
         if (ti.nproj > I.max_nproj) I.max_nproj = ti.nproj;
+        ti.rb = T.rb;
         S->rb.push_back(T.rb);
         for (int a = 0; a < T.natm; ++a) {
             S->atom_xyz.push_back(T.xyz[3 * a]);

@@ -27,7 +27,7 @@ def types_of(system: System):
             sym, f = "Li", "psd_Li.pot"
         else:
             sym, f = "H", "psd_H.pot"
-        out.append(AtomType(sym, ti["Z"], ti["lloc"], os.path.join(PSD, f), sel))
+        out.append(AtomType(sym, ti["Z"], ti["lloc"], os.path.join(PSD, f), sel, rb=ti["rb"]))
     del syms
     return out
 

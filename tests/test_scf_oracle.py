@@ -130,7 +130,20 @@ def check_against_al_log(history):
         assert (h["poisson"][0], "%g" % h["poisson"][1]) == (pits, pres)
 
 
-def test_oracle_scf_converges_like_log(al):
+# results_Al:145-149: atomic forces printed after the SCF (local with its
+# correction term + nonlocal, before symmetrisation), fx fy fz per atom
+AL_FORCES = np.array([
+    [-0.005345683815, -0.012594306453, -0.003231691360],
+    [-0.005219311754, 0.003434469182, 0.011589035826],
+    [0.001435243122, -0.014225977631, 0.013414359697],
+    [0.009239193226, 0.023368210633, -0.021785233269],
+])
+
+
+@pytest.fixture(scope="module")
+def oracle_scf(al):
+    """The oracle's SCF on the Al test (SCF_iteration, src/scf.cpp:84-330):
+    SQ density, energy, AAJ Poisson, LDA potential, Periodic Pulay mixing."""
     S, Rcut, T = al
     osys = H.oracle_for(S, Rcut, T)
     info = S.info
@@ -166,4 +179,40 @@ def test_oracle_scf_converges_like_log(al):
         if err > pp["scf_tol"]:
             veff = vnew
         count += 1
-    check_against_al_log(hist)
+    return {"osys": osys, "history": hist, "veff": veff, "phi": phi, "last": out}
+
+
+def test_oracle_scf_converges_like_log(oracle_scf):
+    check_against_al_log(oracle_scf["history"])
+
+
+def test_oracle_pseudocharge_matches_host_tables(al):
+    """The oracle's ChargeDensity restatement against the library's host
+    pseudocharge: the same arithmetic, so equal bit for bit."""
+    S, Rcut, T = al
+    osys = H.oracle_for(S, Rcut, T)
+    rb = np.array([t.rb for t in H.types_of(S)])
+    cd, pc = osys.charge_density(rb), S.pseudocharge()
+    assert np.array_equal(cd["b"], pc["b"]) and np.array_equal(cd["rho"], pc["rho"])
+    assert (cd["ncharge"], cd["eself"], cd["ecorr"]) == (pc["ncharge"], pc["eself"], pc["ecorr"])
+    # b integrates to -(valence charge); b~ is scaled to the same charge
+    h3 = S.info["h"] ** 3
+    assert abs(cd["ncharge"] + 12) < 1e-3
+    assert abs(np.sum(cd["b_tilda"]) * h3 - cd["ncharge"]) < 1e-10
+
+
+def test_oracle_total_forces_match_log(al, oracle_scf):
+    """Local (ForcesLocal) + nonlocal (ForcesNonLocal) forces at the
+    converged state against the forces the reference printed."""
+    S, Rcut, T = al
+    osys = oracle_scf["osys"]
+    rb = np.array([t.rb for t in H.types_of(S)])
+    cd = osys.charge_density(rb)
+    fl, fc = osys.forces_local(rb, oracle_scf["phi"], cd["b"], cd["b_tilda"], cd["vc"])
+    last = oracle_scf["last"]
+    fnl = osys.forces_nonlocal(oracle_scf["veff"], last["chi"], last["zee"], last["Ci"])
+    # ForcesTotal adds the correction term of ForcesLocal for Correction 1 too
+    # (forces.cpp:613-616)
+    f = fl + fnl + fc
+    # printed with 12 decimals after an 8-rank sum
+    assert np.max(np.abs(f - AL_FORCES)) < 2e-12, np.max(np.abs(f - AL_FORCES))

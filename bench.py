@@ -226,11 +226,20 @@ def main():
         it1 = scf.iteration()
         scf.veff = it1["veff_new"]
         it2 = scf.iteration()
+        log("bench: forces")
+        t0 = time.perf_counter()
+        scf.ForcesLocal()
+        t_loc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        scf.sq.ForcesNonLocal()
+        t_nl = time.perf_counter() - t0
         scf.close()
         out["scf_step"] = {"s": it2["t_sq"] + it2["t_rest"], "sq_s": it2["t_sq"],
                            "energy_poisson_vxc_s": it2["t_rest"], "poisson_iters": it2["poisson"][0],
                            "scf_err": it2["scf_err"], "lambda_f": it2["lambda_f"],
-                           "what": "second SCF iteration from the atomic starting density, unmixed Veff"}
+                           "forces_local_s": t_loc, "forces_nonlocal_s": t_nl,
+                           "what": "second SCF iteration from the atomic starting density, unmixed Veff; "
+                                   "then local and nonlocal forces at that state"}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(S, veff, cfg["Rcut"], cfg["T"], args.cpu_seconds)

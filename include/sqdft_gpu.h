@@ -48,6 +48,7 @@ extern "C" {
 #define SQG_E_OVERFLOW      -11   /* per-node image list larger than the build limit */
 #define SQG_E_NOSTATE       -12   /* forces/getters called before sqg_density */
 #define SQG_E_FDN           -13   /* FD order other than 12 */
+#define SQG_E_SPLINE        -14   /* spline abscissa outside the radial grid (spline.cpp:17-22) */
 #define SQG_W_LANCZOS_MAXIT  1    /* some node hit max_iter=500 (sq.cpp:1073-1074; warning) */
 #define SQG_W_NEWTON_MAXIT   2    /* Newton hit ITMAX=200 (sq.cpp:1479-1482; warning) */
 
@@ -180,6 +181,30 @@ int sqg_vxc_lda(sqg_ctx *ctx, const double *rho, double *vxc);
  * output, ebs/eent from sqg_density, eself/ecorr from the pseudocharge. */
 int sqg_total_energy(sqg_ctx *ctx, const double *phi, const double *b, const double *rho, const double *vxc,
                      double ebs, double eent, double eself, double ecorr, int n_atm, double *etot);
+
+/* The pseudopotential data and rb-replicas ForcesLocal loops over. */
+typedef struct sqg_local_spec {
+    int n_typ;
+    const int *size;              /* per type: radial grid length (>= 2) */
+    const double *const *r;       /* per type: radial grid, increasing from 0 */
+    const double *const *rvloc;   /* per type: r V_loc(r) */
+    const double *const *dvloc;   /* per type: its spline derivative table (getYD_gen) */
+    const double *Z;              /* per type: valence charge */
+    int n_img;                    /* replicas (sqdft_system_rb_images) */
+    const double *img_xyz;        /* 3 per replica (bohr) */
+    const int *img_type, *img_atom;
+    const int *img_box;           /* 6 per replica: intersection lo x,y,z, hi x,y,z */
+    int n_atm;
+} sqg_local_spec;
+
+/* Local forces <- ForcesLocal(DS_SQ*) (src/forces.cpp:20-315, inc/func_sq.h:534)
+ * on one rank: phi from the last Poisson solve, b, b_tilda, vc
+ * from the pseudocharge (n^3 each, x fastest).  floc and fcorr get 3*n_atm
+ * values laid out fx[n_atm], fy[n_atm], fz[n_atm]; ForcesTotal
+ * (src/forces.cpp:613-619) adds floc + fnloc + fcorr.  SQG_E_ARG for an
+ * inconsistent spec, SQG_E_SPLINE when a distance leaves a radial grid. */
+int sqg_forces_local(sqg_ctx *ctx, const sqg_local_spec *spec, const double *phi, const double *b,
+                     const double *b_tilda, const double *vc, double *floc, double *fcorr);
 
 #ifdef __cplusplus
 }

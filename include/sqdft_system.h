@@ -112,10 +112,24 @@ int  sqdft_system_udv(const sqdft_system *S, double *udv);
  * src/initialize.cpp:21-150; first MD step, Correction 1).  Outputs, any of
  * which may be NULL: b[n^3] (x fastest), rho[n^3] = rho_at scaled to hold
  * -Ncharge electrons, ncharge = sum b h^3 (negative), eself, and ecorr,
- * the energy correction (the reference prints it divided by the atom count).
+ * the energy correction (the reference prints it divided by the atom count),
+ * b_tilda[n^3] (the smooth reference charge scaled to Ncharge) and vc[n^3]
+ * (V~ - V shifted as initialize.cpp:752-775), the inputs of ForcesLocal.
  * Returns SQDFT_E_ARG when a type has rb <= 0. */
 int  sqdft_system_pseudocharge(const sqdft_system *S, double *b, double *rho, double *ncharge,
-                               double *eself, double *ecorr);
+                               double *eself, double *ecorr, double *b_tilda, double *vc);
+
+/* Periodic replicas whose rb-box (half-width round(rb/h) nodes around the
+ * nearest node) meets the domain [0, n-1]^3, in Replica_atoms' order (type,
+ * atom, z-shift, y-shift, x-shift; src/initialize.cpp:21-200).  Call with
+ * NULL arrays to get the count in *n_img.  xyz[3*n_img], type[n_img],
+ * atom[n_img] (global index in type order), box[6*n_img] = the intersection
+ * lo x,y,z then hi x,y,z (inclusive node indices). */
+int  sqdft_system_rb_images(const sqdft_system *S, int *n_img, double *xyz, int *type, int *atom,
+                            int *box);
+/* Spline derivative table of type t's r V_loc(r) (getYD_gen, src/spline.cpp:57-126),
+ * `size` doubles: with sqdft_system_radial's r and rVloc it defines V_loc. */
+int  sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d);
 
 /* Poisson-solver and SCF parameters (".input" poisson_tol, poisson_maxiter,
  * beta_aaj, m_aaj, p_aaj, scf_tol, scf_miniter, scf_maxiter, beta_scf,

@@ -27,6 +27,12 @@ class ScfParams(C.Structure):
                 ("scf_maxiter", C.c_int), ("beta_scf", C.c_double), ("m_scf", C.c_int), ("p_scf", C.c_int)]
 
 
+class LocalSpec(C.Structure):
+    _fields_ = [("n_typ", C.c_int), ("size", c_int_p), ("r", C.POINTER(c_dbl_p)), ("rvloc", C.POINTER(c_dbl_p)),
+                ("dvloc", C.POINTER(c_dbl_p)), ("Z", c_dbl_p), ("n_img", C.c_int), ("img_xyz", c_dbl_p),
+                ("img_type", c_int_p), ("img_atom", c_int_p), ("img_box", c_int_p), ("n_atm", C.c_int)]
+
+
 class Spec(C.Structure):
     _fields_ = [("n", C.c_int), ("L", C.c_double), ("h", C.c_double), ("fd_order", C.c_int),
                 ("Rcut", C.c_double), ("npl", C.c_int), ("T", C.c_double),
@@ -86,7 +92,12 @@ SIGNATURES = {
     "sqg_vxc_lda": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p]),
     "sqg_total_energy": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, C.c_double, C.c_double,
                                    C.c_double, C.c_double, C.c_int, c_dbl_p]),
-    "sqdft_system_pseudocharge": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "sqdft_system_pseudocharge": (C.c_int, [C.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                            c_dbl_p]),
+    "sqg_forces_local": (C.c_int, [C.c_void_p, C.POINTER(LocalSpec), c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                   c_dbl_p]),
+    "sqdft_system_rb_images": (C.c_int, [C.c_void_p, c_int_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
+    "sqdft_system_vloc_deriv": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
     "sqdft_system_scf_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sqdft_spline_eval": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int, c_dbl_p, c_dbl_p, C.c_int]),
     "sqdft_spline_derivs": (C.c_int, [c_dbl_p, c_dbl_p, c_dbl_p, C.c_int]),
@@ -113,7 +124,7 @@ STATUS = {
     0: "SQG_OK", -1: "SQG_E_ARG", -2: "SQG_E_HIP", -3: "SQG_E_NOGEOM", -4: "SQG_E_NPL",
     -5: "SQG_E_NEG_RHO", -6: "SQG_E_QL", -7: "SQG_E_BRACKET", -8: "SQG_E_BRENT_ITMAX",
     -9: "SQG_E_DENOM", -10: "SQG_E_CUBE", -11: "SQG_E_OVERFLOW", -12: "SQG_E_NOSTATE",
-    -13: "SQG_E_FDN", 1: "SQG_W_LANCZOS_MAXIT", 2: "SQG_W_NEWTON_MAXIT",
+    -13: "SQG_E_FDN", -14: "SQG_E_SPLINE", 1: "SQG_W_LANCZOS_MAXIT", 2: "SQG_W_NEWTON_MAXIT",
 }
 
 _lib = None

@@ -11,6 +11,7 @@
 #include "sq_kernels.hpp"
 #include "sq_adjoint_kernels.hpp"
 #include "sq_poisson_kernels.hpp"
+#include "sq_local_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -86,6 +87,8 @@ struct sqg_ctx {
     DBuf adj, adj_sgn, adj_dl, adj_wt, adj_in, adj_out, adj_rec;
     // AAJ Poisson: iterates, residual, history, rhs, result, block partials
     DBuf pz_phi, pz_new, pz_res, pz_X, pz_F, pz_rhs, pz_out, pz_DX, pz_DF, pz_part;
+    // local forces: replica boxes, radial tables, block partials, error flag
+    DBuf lf_V, lf_Vt, lf_bJ, lf_bJt, lf_img, lf_R, lf_RV, lf_DV, lf_toff, lf_tsize, lf_Z, lf_part, lf_err;
     bool have_state = false;
     double lambda_f = 0.0;
     // device timing of the hot kernels over the last density call (which:
@@ -1418,6 +1421,127 @@ int sqg_total_energy(sqg_ctx *c, const double *phi, const double *b, const doubl
     E = E + ebs + eent + ecorr;
     *etot = E / n_atm;
     return SQG_OK;
+}
+
+int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, const double *b, const double *bt,
+                     const double *vc, double *floc, double *fcorr)
+{
+    if (!c || !L || !phi || !b || !bt || !vc || !floc || !fcorr) return SQG_E_ARG;
+    const int n = c->p.n, F = c->p.FDn;
+    if (L->n_typ < 1 || L->n_typ > SQDFT_MAX_TYPES || L->n_img < 0 || L->n_atm < 1 || !L->size || !L->r ||
+        !L->rvloc || !L->dvloc || !L->Z || (L->n_img > 0 && (!L->img_xyz || !L->img_type || !L->img_atom || !L->img_box)))
+        return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    // radial tables of all types side by side
+    std::vector<int> toff(L->n_typ), tsize(L->n_typ);
+    std::vector<double> R, RV, DV;
+    for (int t = 0; t < L->n_typ; ++t) {
+        if (L->size[t] < 2 || !L->r[t] || !L->rvloc[t] || !L->dvloc[t]) return SQG_E_ARG;
+        toff[t] = (int)R.size();
+        tsize[t] = L->size[t];
+        R.insert(R.end(), L->r[t], L->r[t] + L->size[t]);
+        RV.insert(RV.end(), L->rvloc[t], L->rvloc[t] + L->size[t]);
+        DV.insert(DV.end(), L->dvloc[t], L->dvloc[t] + L->size[t]);
+    }
+    // replicas: every box index must lie in [0, n-1] (the kernels gather
+    // phi, b, b~, vc there without wrapping)
+    std::vector<sqk::LImg> img(L->n_img);
+    std::vector<long> npts(L->n_img);
+    for (int m = 0; m < L->n_img; ++m) {
+        sqk::LImg &I = img[m];
+        I.x = L->img_xyz[3 * m]; I.y = L->img_xyz[3 * m + 1]; I.z = L->img_xyz[3 * m + 2];
+        I.type = L->img_type[m]; I.pad = 0;
+        if (I.type < 0 || I.type >= L->n_typ || L->img_atom[m] < 0 || L->img_atom[m] >= L->n_atm) return SQG_E_ARG;
+        long np = 1;
+        for (int d = 0; d < 3; ++d) {
+            I.lo[d] = L->img_box[6 * m + d];
+            I.hi[d] = L->img_box[6 * m + 3 + d];
+            if (I.lo[d] < 0 || I.hi[d] > n - 1 || I.lo[d] > I.hi[d]) return SQG_E_ARG;
+            np *= I.hi[d] - I.lo[d] + 1 + 4 * F;
+        }
+        npts[m] = np;
+    }
+    const size_t N = (size_t)n * n * n;
+    // batches of replicas whose boxes fit the scratch (64M points per array)
+    const long CAP = 1L << 26;
+    long maxb = 1;
+    for (long v : npts) maxb = std::max(maxb, v);
+    const long cap = std::max(CAP, maxb);
+    const int GX = 64;
+    for (DBuf *d : {&c->lf_V, &c->lf_Vt, &c->lf_bJ, &c->lf_bJt}) HIPCHK(d->ensure((size_t)cap * sizeof(double)));
+    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X}) HIPCHK(d->ensure(N * sizeof(double)));
+    HIPCHK(c->lf_R.ensure(R.size() * sizeof(double)));
+    HIPCHK(c->lf_RV.ensure(R.size() * sizeof(double)));
+    HIPCHK(c->lf_DV.ensure(R.size() * sizeof(double)));
+    HIPCHK(c->lf_toff.ensure(L->n_typ * sizeof(int)));
+    HIPCHK(c->lf_tsize.ensure(L->n_typ * sizeof(int)));
+    HIPCHK(c->lf_Z.ensure(L->n_typ * sizeof(double)));
+    HIPCHK(c->lf_img.ensure(std::max<size_t>(img.size(), 1) * sizeof(sqk::LImg)));
+    HIPCHK(c->lf_err.ensure(sizeof(int)));
+    HIPCHK(hipMemcpyAsync(c->lf_R.p, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_RV.p, RV.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_DV.p, DV.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_toff.p, toff.data(), L->n_typ * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_tsize.p, tsize.data(), L->n_typ * sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_Z.p, L->Z, L->n_typ * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_phi.p, phi, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_new.p, b, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_res.p, bt, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->pz_X.p, vc, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->lf_err.p, 0, sizeof(int), c->st));
+
+    sqk::LocArgs A{};
+    A.n = n; A.F = F; A.h = c->p.h; A.h3 = std::pow(c->p.h, 3);
+    for (int a = 0; a <= F; ++a) { A.lap[a] = c->p.coeff_lap[a]; A.grad[a] = c->p.coeff_grad[a]; }
+    A.R = c->lf_R.as<double>(); A.RV = c->lf_RV.as<double>(); A.DV = c->lf_DV.as<double>();
+    A.toff = c->lf_toff.as<int>(); A.tsize = c->lf_tsize.as<int>(); A.Z = c->lf_Z.as<double>();
+    A.V = c->lf_V.as<double>(); A.Vt = c->lf_Vt.as<double>(); A.bJ = c->lf_bJ.as<double>();
+    A.bJt = c->lf_bJt.as<double>();
+    A.phi = c->pz_phi.as<double>(); A.b = c->pz_new.as<double>(); A.bt = c->pz_res.as<double>();
+    A.vc = c->pz_X.as<double>();
+    A.err = c->lf_err.as<int>();
+    for (int q = 0; q < 3 * L->n_atm; ++q) floc[q] = fcorr[q] = 0.0;
+    std::vector<double> part;
+    for (int m0 = 0; m0 < L->n_img;) {
+        int m1 = m0;
+        long used = 0, bmax = 1;
+        while (m1 < L->n_img && m1 - m0 < 65535 && used + npts[m1] <= cap) {
+            img[m1].off = used;
+            used += npts[m1];
+            bmax = std::max(bmax, npts[m1]);
+            ++m1;
+        }
+        const int nb = m1 - m0;
+        HIPCHK(hipMemcpyAsync(c->lf_img.p, img.data() + m0, nb * sizeof(sqk::LImg), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c->lf_part.ensure((size_t)nb * GX * 6 * sizeof(double)));
+        A.img = c->lf_img.as<sqk::LImg>();
+        A.nimg = nb;
+        A.part = c->lf_part.as<double>();
+        const int gx = (int)std::min<long>((bmax + sqk::PB - 1) / sqk::PB, GX);
+        hipLaunchKernelGGL(sqk::k_loc_potential, dim3(gx, nb), dim3(sqk::PB), 0, c->st, A);
+        hipLaunchKernelGGL(sqk::k_loc_charge, dim3(gx, nb), dim3(sqk::PB), 0, c->st, A);
+        hipLaunchKernelGGL(sqk::k_loc_force, dim3(GX, nb), dim3(sqk::PB), 0, c->st, A);
+        HIPCHK(hipGetLastError());
+        part.resize((size_t)nb * GX * 6);
+        HIPCHK(hipMemcpyAsync(part.data(), c->lf_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        // replica totals in block order, added to the atom in replica order
+        for (int q = 0; q < nb; ++q) {
+            double f[6] = {0, 0, 0, 0, 0, 0};
+            for (int bx = 0; bx < GX; ++bx)
+                for (int k = 0; k < 6; ++k) f[k] += part[((size_t)q * GX + bx) * 6 + k];
+            const int a = L->img_atom[m0 + q];
+            for (int d = 0; d < 3; ++d) {
+                floc[d * L->n_atm + a] += f[d];
+                fcorr[d * L->n_atm + a] += f[3 + d];
+            }
+        }
+        m0 = m1;
+    }
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c->lf_err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return err ? SQG_E_SPLINE : SQG_OK;
 }
 
 }  // extern "C"

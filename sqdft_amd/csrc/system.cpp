@@ -454,6 +454,50 @@ struct Tokens {
     }
 };
 
+// Periodic replicas whose rb-box meets the domain [0, n-1]^3 (Replica_atoms,
+// src/initialize.cpp:21-200, one rank), in its (type, atom, z, y, x) order.
+struct RbImage { double x, y, z; int lo[3], hi[3]; int type, atom; };
+
+int rb_image_list(const sqdft_system *S, std::vector<RbImage> &out)
+{
+    const sqdft_info &I = S->info;
+    const int n = I.n;
+    const double h = I.h, L = I.L;
+    out.clear();
+    size_t ia = 0;
+    for (int t = 0; t < I.n_typ; ++t) {
+        const double rb = S->rb[t];
+        if (!(rb > 0)) return SQDFT_E_ARG;
+        const int Rb = (int)std::round(rb / h);
+        for (int a = 0; a < S->tinfo[t].natm; ++a, ++ia) {
+            const double c[3] = {S->atom_xyz[3 * ia], S->atom_xyz[3 * ia + 1], S->atom_xyz[3 * ia + 2]};
+            int pm[3], mm[3];
+            for (int d = 0; d < 3; ++d) {
+                pm[d] = (int)std::floor(((L + rb - c[d]) / L) + kTinyR);
+                mm[d] = (int)std::ceil(((-rb - c[d]) / L) - kTinyR);
+            }
+            for (int rr = mm[2]; rr <= pm[2]; ++rr)
+                for (int qq = mm[1]; qq <= pm[1]; ++qq)
+                    for (int pp = mm[0]; pp <= pm[0]; ++pp) {
+                        RbImage R;
+                        R.x = c[0] + pp * L; R.y = c[1] + qq * L; R.z = c[2] + rr * L;
+                        const double A[3] = {R.x, R.y, R.z};
+                        bool meet = true;
+                        for (int d = 0; d < 3; ++d) {
+                            const int nn = nearest_node(A[d], h);
+                            R.lo[d] = std::max(nn - Rb, 0);
+                            R.hi[d] = std::min(nn + Rb, n - 1);
+                            meet = meet && R.lo[d] <= R.hi[d];
+                        }
+                        if (!meet) continue;
+                        R.type = t; R.atom = (int)ia;
+                        out.push_back(R);
+                    }
+        }
+    }
+    return SQDFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -684,7 +728,7 @@ int sqdft_system_scf_params(const sqdft_system *S, sqdft_scf_params *pp)
 // the reference's order term by term so that the printed energy correction
 // is reproduced to its last digit.
 int sqdft_system_pseudocharge(const sqdft_system *S, double *b_out, double *rho_out, double *ncharge,
-                              double *eself, double *ecorr)
+                              double *eself, double *ecorr, double *bt_out, double *vc_out)
 {
     if (!S) return SQDFT_E_ARG;
     const sqdft_info &I = S->info;
@@ -697,87 +741,68 @@ int sqdft_system_pseudocharge(const sqdft_system *S, double *b_out, double *rho_
     const double rc_t = 0.5;   // rc_tilda
     const double k4pi = -1 / (4 * M_PI);
     double Eself = 0, Ncharge = 0, Nel = 0, Eself_t = 0, Ncharge_t = 0, Vcon = 0;
-    size_t ia = 0;   // atom index in type order
-    for (int t = 0; t < I.n_typ; ++t) {
-        const PsdTable &P = S->psd[t];
-        Spline sUU;
-        sUU.build(P.r.data(), P.uu.data(), P.size);
-        const double rb = S->rb[t];
-        const int Rb = (int)std::round(rb / h);
+    std::vector<RbImage> images;
+    if (int st = rb_image_list(S, images)) return st;
+    std::vector<Spline> sUU(I.n_typ);
+    for (int t = 0; t < I.n_typ; ++t) sUU[t].build(S->psd[t].r.data(), S->psd[t].uu.data(), S->psd[t].size);
+    for (const RbImage &R : images) {
+        const PsdTable &P = S->psd[R.type];
         const double Z = (double)P.Z;
-        for (int a = 0; a < S->tinfo[t].natm; ++a, ++ia) {
-            const double x0 = S->atom_xyz[3 * ia], y0 = S->atom_xyz[3 * ia + 1], z0 = S->atom_xyz[3 * ia + 2];
-            const int px = (int)std::floor(((L + rb - x0) / L) + kTinyR);
-            const int py = (int)std::floor(((L + rb - y0) / L) + kTinyR);
-            const int pz = (int)std::floor(((L + rb - z0) / L) + kTinyR);
-            const int mx = (int)std::ceil(((-rb - x0) / L) - kTinyR);
-            const int my = (int)std::ceil(((-rb - y0) / L) - kTinyR);
-            const int mz = (int)std::ceil(((-rb - z0) / L) - kTinyR);
-            for (int rr = mz; rr <= pz; ++rr)
-                for (int qq = my; qq <= py; ++qq)
-                    for (int pp = mx; pp <= px; ++pp) {
-                        const double Ax = x0 + pp * L, Ay = y0 + qq * L, Az = z0 + rr * L;
-                        const int inn = nearest_node(Ax, h), jnn = nearest_node(Ay, h), knn = nearest_node(Az, h);
-                        // intersection of the rb-box with [0, n-1]^3
-                        const int xs = std::max(inn - Rb, 0), xe = std::min(inn + Rb, n - 1);
-                        const int ys = std::max(jnn - Rb, 0), ye = std::min(jnn + Rb, n - 1);
-                        const int zs = std::max(knn - Rb, 0), ze = std::min(knn + Rb, n - 1);
-                        if (xs > xe || ys > ye || zs > ze) continue;
-                        const int nx = xe - xs + 1 + 2 * F, ny = ye - ys + 1 + 2 * F, nz = ze - zs + 1 + 2 * F;
-                        const int i0 = xs - F, j0 = ys - F, k0 = zs - F;
-                        std::vector<double> V((size_t)nx * ny * nz), Vt((size_t)nx * ny * nz);
-                        auto at = [&](int k, int j, int i) { return ((size_t)k * ny + j) * nx + i; };
-                        for (int k = 0; k < nz; ++k)
-                            for (int j = 0; j < ny; ++j)
-                                for (int i = 0; i < nx; ++i) {
-                                    const double dx = (i0 + i) * h - Ax, dy = (j0 + j) * h - Ay, dz = (k0 + k) * h - Az;
-                                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
-                                    double v = 0;
-                                    if (!P.sVloc.eval(r, &v)) return SQDFT_E_SPLINE;
-                                    if (std::fabs(r - 0.0) < kTinyR) v = P.rVloc[1] / P.r[1];
-                                    else v = v / r;
-                                    V[at(k, j, i)] = v;
-                                    if (i >= F && i < nx - F && j >= F && j < ny - F && k >= F && k < nz - F) {
-                                        double u = 0;
-                                        if (!sUU.eval(r, &u)) return SQDFT_E_SPLINE;
-                                        rho_at[((size_t)(k + k0) * n + (j + j0)) * n + (i + i0)] += u;
-                                        Nel += u * h * h * h;
-                                    }
-                                    if (r < rc_t)
-                                        Vt[at(k, j, i)] = -Z * (9 * std::pow(r, 7) - 30 * rc_t * std::pow(r, 6) +
-                                                                28 * rc_t * rc_t * std::pow(r, 5) -
-                                                                14 * std::pow(rc_t, 5) * std::pow(r, 2) + 12 * std::pow(rc_t, 7)) /
-                                                          (5 * std::pow(rc_t, 8));
-                                    else
-                                        Vt[at(k, j, i)] = -Z / r;
-                                }
-                        for (int k = zs; k <= ze; ++k)
-                            for (int j = ys; j <= ye; ++j)
-                                for (int i = xs; i <= xe; ++i) {
-                                    const int lk = k - k0, lj = j - j0, li = i - i0;
-                                    double bv = V[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
-                                    double btv = Vt[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
-                                    for (int s = 1; s <= F; ++s) {
-                                        bv += (V[at(lk, lj, li - s)] + V[at(lk, lj, li + s)] + V[at(lk, lj - s, li)] +
-                                               V[at(lk, lj + s, li)] + V[at(lk - s, lj, li)] + V[at(lk + s, lj, li)]) *
-                                              I.coeff_lap[s] * k4pi;
-                                        btv += (Vt[at(lk, lj, li - s)] + Vt[at(lk, lj, li + s)] + Vt[at(lk, lj - s, li)] +
-                                                Vt[at(lk, lj + s, li)] + Vt[at(lk - s, lj, li)] + Vt[at(lk + s, lj, li)]) *
-                                               I.coeff_lap[s] * k4pi;
-                                    }
-                                    const size_t g = ((size_t)k * n + j) * n + i;
-                                    b[g] += bv;
-                                    Eself += 0.5 * bv * V[at(lk, lj, li)] * h * h * h;
-                                    Ncharge += bv * h * h * h;
-                                    bt[g] += btv;
-                                    Eself_t += 0.5 * btv * Vt[at(lk, lj, li)] * h * h * h;
-                                    Ncharge_t += btv * h * h * h;
-                                    Vc[g] += Vt[at(lk, lj, li)] - V[at(lk, lj, li)];
-                                }
-                        if (xs <= 0 && xe >= 0 && ys <= 0 && ye >= 0 && zs <= 0 && ze >= 0)
-                            Vcon += Vt[at(0 - k0, 0 - j0, 0 - i0)] - V[at(0 - k0, 0 - j0, 0 - i0)];
+        const double Ax = R.x, Ay = R.y, Az = R.z;
+        const int xs = R.lo[0], xe = R.hi[0], ys = R.lo[1], ye = R.hi[1], zs = R.lo[2], ze = R.hi[2];
+        const int nx = xe - xs + 1 + 2 * F, ny = ye - ys + 1 + 2 * F, nz = ze - zs + 1 + 2 * F;
+        const int i0 = xs - F, j0 = ys - F, k0 = zs - F;
+        std::vector<double> V((size_t)nx * ny * nz), Vt((size_t)nx * ny * nz);
+        auto at = [&](int k, int j, int i) { return ((size_t)k * ny + j) * nx + i; };
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const double dx = (i0 + i) * h - Ax, dy = (j0 + j) * h - Ay, dz = (k0 + k) * h - Az;
+                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    double v = 0;
+                    if (!P.sVloc.eval(r, &v)) return SQDFT_E_SPLINE;
+                    if (std::fabs(r - 0.0) < kTinyR) v = P.rVloc[1] / P.r[1];
+                    else v = v / r;
+                    V[at(k, j, i)] = v;
+                    if (i >= F && i < nx - F && j >= F && j < ny - F && k >= F && k < nz - F) {
+                        double u = 0;
+                        if (!sUU[R.type].eval(r, &u)) return SQDFT_E_SPLINE;
+                        rho_at[((size_t)(k + k0) * n + (j + j0)) * n + (i + i0)] += u;
+                        Nel += u * h * h * h;
                     }
-        }
+                    if (r < rc_t)
+                        Vt[at(k, j, i)] = -Z * (9 * std::pow(r, 7) - 30 * rc_t * std::pow(r, 6) +
+                                                28 * rc_t * rc_t * std::pow(r, 5) -
+                                                14 * std::pow(rc_t, 5) * std::pow(r, 2) + 12 * std::pow(rc_t, 7)) /
+                                          (5 * std::pow(rc_t, 8));
+                    else
+                        Vt[at(k, j, i)] = -Z / r;
+                }
+        for (int k = zs; k <= ze; ++k)
+            for (int j = ys; j <= ye; ++j)
+                for (int i = xs; i <= xe; ++i) {
+                    const int lk = k - k0, lj = j - j0, li = i - i0;
+                    double bv = V[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
+                    double btv = Vt[at(lk, lj, li)] * 3 * I.coeff_lap[0] * k4pi;
+                    for (int s = 1; s <= F; ++s) {
+                        bv += (V[at(lk, lj, li - s)] + V[at(lk, lj, li + s)] + V[at(lk, lj - s, li)] +
+                               V[at(lk, lj + s, li)] + V[at(lk - s, lj, li)] + V[at(lk + s, lj, li)]) *
+                              I.coeff_lap[s] * k4pi;
+                        btv += (Vt[at(lk, lj, li - s)] + Vt[at(lk, lj, li + s)] + Vt[at(lk, lj - s, li)] +
+                                Vt[at(lk, lj + s, li)] + Vt[at(lk - s, lj, li)] + Vt[at(lk + s, lj, li)]) *
+                               I.coeff_lap[s] * k4pi;
+                    }
+                    const size_t g = ((size_t)k * n + j) * n + i;
+                    b[g] += bv;
+                    Eself += 0.5 * bv * V[at(lk, lj, li)] * h * h * h;
+                    Ncharge += bv * h * h * h;
+                    bt[g] += btv;
+                    Eself_t += 0.5 * btv * Vt[at(lk, lj, li)] * h * h * h;
+                    Ncharge_t += btv * h * h * h;
+                    Vc[g] += Vt[at(lk, lj, li)] - V[at(lk, lj, li)];
+                }
+        if (xs <= 0 && xe >= 0 && ys <= 0 && ye >= 0 && zs <= 0 && ze >= 0)
+            Vcon += Vt[at(0 - k0, 0 - j0, 0 - i0)] - V[at(0 - k0, 0 - j0, 0 - i0)];
     }
     // rho_guess = rho_at scaled to -Ncharge electrons; b~ scaled to Ncharge
     double Nel2 = 0;
@@ -797,6 +822,33 @@ int sqdft_system_pseudocharge(const sqdft_system *S, double *b_out, double *rho_
     if (ncharge) *ncharge = Ncharge;
     if (eself) *eself = Eself;
     if (ecorr) *ecorr = Ec + Eself - Eself_t;
+    if (bt_out) std::memcpy(bt_out, bt.data(), N3 * sizeof(double));
+    if (vc_out) std::memcpy(vc_out, Vc.data(), N3 * sizeof(double));
+    return SQDFT_OK;
+}
+
+int sqdft_system_rb_images(const sqdft_system *S, int *n_img, double *xyz, int *type, int *atom, int *box)
+{
+    if (!S || !n_img) return SQDFT_E_ARG;
+    std::vector<RbImage> im;
+    const int st = rb_image_list(S, im);
+    if (st) return st;
+    for (size_t m = 0; m < im.size(); ++m) {
+        const RbImage &R = im[m];
+        if (xyz) { xyz[3 * m] = R.x; xyz[3 * m + 1] = R.y; xyz[3 * m + 2] = R.z; }
+        if (type) type[m] = R.type;
+        if (atom) atom[m] = R.atom;
+        if (box) for (int d = 0; d < 3; ++d) { box[6 * m + d] = R.lo[d]; box[6 * m + 3 + d] = R.hi[d]; }
+    }
+    *n_img = (int)im.size();
+    return SQDFT_OK;
+}
+
+int sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d)
+{
+    if (!S || !d || t < 0 || t >= S->info.n_typ) return SQDFT_E_ARG;
+    const PsdTable &P = S->psd[t];
+    spline_derivs(P.r.data(), P.rVloc.data(), d, P.size);
     return SQDFT_OK;
 }
 

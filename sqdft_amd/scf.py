@@ -47,6 +47,7 @@ class SCF:
         pc = system.pseudocharge()
         self.b, self.rho = pc["b"], pc["rho"]
         self.ncharge, self.eself, self.ecorr = pc["ncharge"], pc["eself"], pc["ecorr"]
+        self.b_tilda, self.vc = pc["b_tilda"], pc["vc"]
         self.pp = system.poisson_params()
         self.n3 = system.n ** 3
         self.phi = None
@@ -84,6 +85,34 @@ class SCF:
         L.check("sqg_total_energy", self._lib.sqg_total_energy(
             self.sq._ctx, *[L.dptr(a) for a in arrs], ebs, eent, self.eself, self.ecorr, self.S.n_atm, C.byref(e)))
         return e.value
+
+    def ForcesLocal(self) -> tuple[np.ndarray, np.ndarray]:
+        """Local forces and their correction term, each (n_atm, 3), from the
+        current phi and the pseudocharge (ForcesLocal, src/forces.cpp:20-315)."""
+        S, nt, na = self.S, self.S.info["n_typ"], self.S.n_atm
+        tabs = [S.vloc_spline(t) for t in range(nt)]
+        keep = [np.ascontiguousarray(x, dtype=np.float64) for tb in tabs for x in (tb["r"], tb["rVloc"], tb["d"])]
+        ptrs = lambda k: (L.c_dbl_p * nt)(*[L.dptr(keep[3 * t + k]) for t in range(nt)])  # noqa: E731
+        size = np.array([tb["r"].size for tb in tabs], dtype=np.int32)
+        Z = np.array([float(S.type_info(t)["Z"]) for t in range(nt)])
+        im = S.rb_images()
+        xyz = np.ascontiguousarray(im["xyz"], dtype=np.float64).reshape(-1)
+        typ, atm = np.ascontiguousarray(im["type"]), np.ascontiguousarray(im["atom"])
+        box = np.ascontiguousarray(im["box"], dtype=np.int32).reshape(-1)
+        spec = L.LocalSpec(n_typ=nt, size=L.iptr(size), r=ptrs(0), rvloc=ptrs(1), dvloc=ptrs(2), Z=L.dptr(Z),
+                           n_img=typ.size, img_xyz=L.dptr(xyz), img_type=L.iptr(typ), img_atom=L.iptr(atm),
+                           img_box=L.iptr(box), n_atm=na)
+        arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (self.phi, self.b, self.b_tilda, self.vc)]
+        fl, fc = np.zeros(3 * na), np.zeros(3 * na)
+        L.check("sqg_forces_local", self._lib.sqg_forces_local(self.sq._ctx, C.byref(spec), *[L.dptr(a) for a in arrs],
+                                                               L.dptr(fl), L.dptr(fc)))
+        return fl.reshape(3, na).T.copy(), fc.reshape(3, na).T.copy()
+
+    def ForcesTotal(self) -> np.ndarray:
+        """Atomic forces (n_atm, 3) after the SCF: local + nonlocal + the
+        correction term, before the net-force removal (src/forces.cpp:580-647)."""
+        fl, fc = self.ForcesLocal()
+        return fl + self.sq.ForcesNonLocal() + fc
 
     # ------------------------------------------------------------ driver
     def initial_potential(self) -> np.ndarray:

@@ -140,14 +140,37 @@ class System:
 
     def pseudocharge(self) -> dict:
         """Pseudocharge b, starting density rho (rho_at scaled to -Ncharge
-        electrons), Ncharge, Eself and the energy correction Ecorr
-        (ChargeDensity, src/initialize.cpp:471-784); arrays are n^3, x fastest."""
+        electrons), Ncharge, Eself, the energy correction Ecorr, and b_tilda,
+        vc for the local forces (ChargeDensity, src/initialize.cpp:471-784);
+        arrays are n^3, x fastest."""
         n3 = self.n ** 3
-        b = np.zeros(n3); rho = np.zeros(n3)
+        b, rho, bt, vc = np.zeros(n3), np.zeros(n3), np.zeros(n3), np.zeros(n3)
         nc = C.c_double(); es = C.c_double(); ec = C.c_double()
         L.check("sqdft_system_pseudocharge", self._lib.sqdft_system_pseudocharge(
-            self._h, L.dptr(b), L.dptr(rho), C.byref(nc), C.byref(es), C.byref(ec)))
-        return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value}
+            self._h, L.dptr(b), L.dptr(rho), C.byref(nc), C.byref(es), C.byref(ec), L.dptr(bt), L.dptr(vc)))
+        return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value,
+                "b_tilda": bt, "vc": vc}
+
+    def rb_images(self) -> dict:
+        """Replicas whose rb-box meets the domain (Replica_atoms,
+        src/initialize.cpp:21-200): xyz, type, atom, box (lo xyz, hi xyz)."""
+        cnt = C.c_int()
+        null = C.POINTER(C.c_double)(), C.POINTER(C.c_int)()
+        L.check("sqdft_system_rb_images", self._lib.sqdft_system_rb_images(
+            self._h, C.byref(cnt), null[0], null[1], null[1], null[1]))
+        m = cnt.value
+        xyz = np.zeros(3 * m); typ = np.zeros(m, np.int32); atm = np.zeros(m, np.int32)
+        box = np.zeros(6 * m, np.int32)
+        L.check("sqdft_system_rb_images", self._lib.sqdft_system_rb_images(
+            self._h, C.byref(cnt), L.dptr(xyz), L.iptr(typ), L.iptr(atm), L.iptr(box)))
+        return {"xyz": xyz.reshape(m, 3), "type": typ, "atom": atm, "box": box.reshape(m, 6)}
+
+    def vloc_spline(self, t: int) -> dict:
+        """r, r V_loc(r) and its spline derivative table for type t."""
+        rad = self.radial(t)
+        d = np.zeros(rad["r"].size)
+        L.check("sqdft_system_vloc_deriv", self._lib.sqdft_system_vloc_deriv(self._h, t, L.dptr(d)))
+        return {"r": rad["r"], "rVloc": rad["rVloc"], "d": d}
 
     def poisson_params(self) -> dict:
         """AAJ and SCF parameters of the input: tol, maxiter, beta, m, p of the

@@ -88,3 +88,40 @@ def test_scf_on_gpu_converges_like_log(al):
     check_against_al_log(out["history"])
     assert abs(out["free_energy"] - (-2.453313155422)) < 3e-12
     scf.close()
+
+
+def test_total_forces_on_gpu_match_log(al):
+    """ForcesTotal after the GPU SCF: sqg_forces_local + sqg_forces_nonlocal
+    against the forces the reference printed (results_Al:145-149), and the
+    local terms against the oracle's ForcesLocal on the same phi."""
+    from test_scf_oracle import AL_FORCES
+    S, Rcut, T = al
+    say("forces: SCF + total forces on the GPU path")
+    scf = SCF(S)
+    out = scf.run()
+    assert out["converged"]
+    f = scf.ForcesTotal()
+    say("max |f - log|", float(np.max(np.abs(f - AL_FORCES))))
+    assert np.max(np.abs(f - AL_FORCES)) < 1e-10
+    osys = H.oracle_for(S, Rcut, T)
+    rb = np.array([t.rb for t in H.types_of(S)])
+    fl_o, fc_o = osys.forces_local(rb, scf.phi, scf.b, scf.b_tilda, scf.vc)
+    fl, fc = scf.ForcesLocal()
+    assert np.max(np.abs(fl - fl_o)) < 1e-13 and np.max(np.abs(fc - fc_o)) < 1e-13
+    scf.close()
+
+
+def test_local_forces_two_types_match_oracle():
+    """LiH (two types, different radial grids and rb): the GPU local forces on
+    the starting phi against the oracle."""
+    S, Rcut, T = H.test_lih()
+    say("forces: LiH local forces vs oracle")
+    scf = SCF(S)
+    scf.initial_potential()
+    fl, fc = scf.ForcesLocal()
+    osys = H.oracle_for(S, Rcut, T)
+    rb = np.array([t.rb for t in H.types_of(S)])
+    fl_o, fc_o = osys.forces_local(rb, scf.phi, scf.b, scf.b_tilda, scf.vc)
+    scale = np.max(np.abs(fl_o)) + np.max(np.abs(fc_o))
+    assert np.max(np.abs(fl - fl_o)) < 1e-12 * scale and np.max(np.abs(fc - fc_o)) < 1e-12 * scale
+    scf.close()

@@ -240,6 +240,25 @@ def main():
                            "forces_local_s": t_loc, "forces_nonlocal_s": t_nl,
                            "what": "second SCF iteration from the atomic starting density, unmixed Veff; "
                                    "then local and nonlocal forces at that state"}
+    if not args.no_scf:
+        # SCF iterations to SCF_err 1e-6 on the reference's Al test input:
+        # its Periodic Pulay mixing vs Newton steps from the adjoint response
+        log("bench: scf mixing on the Al test input")
+        from sqdft_amd.scf import SCF
+        from sqdft_amd.system import System
+        A = System.from_files(os.path.join(ROOT, "tests", "golden", "Al"), "test_Al")
+        mix = {}
+        for name, kw in (("pulay", {}), ("response", {"mixing": "response"})):
+            sc = SCF(A)
+            t0 = time.perf_counter()
+            r = sc.run(scf_tol=1e-6, **kw)
+            mix[name] = {"iterations": r["iterations"], "converged": r["converged"],
+                         "response_products": r["response_products"], "s": time.perf_counter() - t0,
+                         "free_energy": r["free_energy"]}
+            sc.close()
+        A.close()
+        out["scf_mixing"] = {"workload": "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6",
+                             **mix}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(S, veff, cfg["Rcut"], cfg["T"], args.cpu_seconds)

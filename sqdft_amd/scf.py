@@ -24,6 +24,32 @@ from .sq import SpectralQuadrature
 from .system import System
 
 
+def gmres(matvec, rhs: np.ndarray, k: int) -> tuple[np.ndarray, float]:
+    """k steps of GMRES from x0 = 0 (modified Gram-Schmidt Arnoldi, small
+    least-squares solve); returns x and the relative residual."""
+    beta0 = float(np.linalg.norm(rhs))
+    if beta0 == 0.0:
+        return np.zeros_like(rhs), 0.0
+    Q = [rhs / beta0]
+    Hm = np.zeros((k + 1, k))
+    kk = k
+    for j in range(k):
+        w = matvec(Q[j])
+        for i in range(j + 1):
+            Hm[i, j] = Q[i] @ w
+            w = w - Hm[i, j] * Q[i]
+        Hm[j + 1, j] = np.linalg.norm(w)
+        if Hm[j + 1, j] <= 1e-14 * beta0:      # invariant subspace: exact solve
+            kk = j + 1
+            break
+        Q.append(w / Hm[j + 1, j])
+    e1 = np.zeros(kk + 1)
+    e1[0] = beta0
+    y = np.linalg.lstsq(Hm[: kk + 1, :kk], e1, rcond=None)[0]
+    res = float(np.linalg.norm(Hm[: kk + 1, :kk] @ y - e1)) / beta0
+    return sum(y[i] * Q[i] for i in range(kk)), res
+
+
 def anderson_extrapolation(DX: np.ndarray, DF: np.ndarray, res: np.ndarray, beta: float) -> np.ndarray:
     """Anderson update of AndersonExtrapolation (src/anderson.cpp:20-100):
     x = pinv(DF DF') DF res (exactly zero singular values dropped, as
@@ -60,16 +86,17 @@ class SCF:
         self.sq.close()
 
     # ------------------------------------------------------------ kernels
-    def PoissonSolver_AAJ(self, rhs, phi_guess) -> np.ndarray:
+    def PoissonSolver_AAJ(self, rhs, phi_guess, tol: float | None = None, log: bool = True) -> np.ndarray:
         r = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
         g = np.ascontiguousarray(phi_guess, dtype=np.float64).reshape(-1)
         phi = np.zeros(self.n3)
         it, rr = C.c_int(), C.c_double()
         pp = self.pp
         L.check("sqg_poisson_aaj", self._lib.sqg_poisson_aaj(
-            self.sq._ctx, L.dptr(r), L.dptr(g), pp["tol"], pp["maxiter"], pp["beta"], pp["m"], pp["p"],
-            L.dptr(phi), C.byref(it), C.byref(rr)))
-        self.poisson_log.append((it.value, rr.value))
+            self.sq._ctx, L.dptr(r), L.dptr(g), pp["tol"] if tol is None else tol, pp["maxiter"], pp["beta"],
+            pp["m"], pp["p"], L.dptr(phi), C.byref(it), C.byref(rr)))
+        if log:
+            self.poisson_log.append((it.value, rr.value))
         return phi
 
     def ExchangeCorrelationPotential(self, rho) -> np.ndarray:
@@ -114,6 +141,32 @@ class SCF:
         fl, fc = self.ForcesLocal()
         return fl + self.sq.ForcesNonLocal() + fc
 
+    # ------------------------------------------------------------ response
+    def vxc_derivative(self, rho, eta: float = 1e-4) -> np.ndarray:
+        """dVxc/drho per grid point, by a central difference in log rho on
+        the device LDA kernel (relative step eta, error O(eta^2))."""
+        r = np.abs(np.asarray(rho, dtype=np.float64).reshape(-1))
+        return (self.ExchangeCorrelationPotential(r * (1 + eta)) -
+                self.ExchangeCorrelationPotential(r * (1 - eta))) / (2 * eta * r)
+
+    def response_step(self, res, rho, krylov: int = 4, products: str = "adjoint") -> tuple[np.ndarray, float]:
+        """Newton step x of the SCF fixed point Veff = G(rho(Veff)), G = the
+        Poisson + LDA map: solves (I - G' chi) x = res with `krylov` GMRES
+        steps.  chi v is the fixed-N density response at the last density
+        step, from the hand-written adjoint (chi^T v, sqg_adjoint_veff; chi of
+        the truncated cubes is symmetric up to the truncation) or the
+        tangent (chi v, sqg_tangent_veff); G' d = Poisson(d) + Vxc'(rho) d."""
+        D = self.vxc_derivative(rho)
+        zero = np.zeros(self.n3)
+        resp = self.sq.adjoint_veff if products == "adjoint" else self.sq.tangent_veff
+
+        def mv(v):
+            drho = resp(v, fixed_n=True)
+            dphi = self.PoissonSolver_AAJ(drho, zero, tol=1e-8, log=False)
+            return v - (dphi + D * drho)
+
+        return gmres(mv, np.asarray(res, dtype=np.float64).reshape(-1), krylov)
+
     # ------------------------------------------------------------ driver
     def initial_potential(self) -> np.ndarray:
         """phi from rho_at + b with phi_guess = 1, Vxc(rho_at), Veff (src/scf.cpp:353-372)."""
@@ -141,12 +194,19 @@ class SCF:
                 "ebs": out["ebs"], "eent": out["eent"], "stats": out["stats"],
                 "poisson": self.poisson_log[-1], "t_sq": t1 - t0, "t_rest": t2 - t1}
 
-    def run(self) -> dict:
+    def run(self, mixing: str = "pulay", scf_tol: float | None = None, krylov: int = 4,
+            products: str = "adjoint") -> dict:
         """SCF loop to scf_tol (SCF_iteration, src/scf.cpp:84-330): per
-        iteration the SQ density, energy, Poisson and Vxc, then Veff mixed as
-        Veff_old + beta_scf res with an Anderson step every p_scf iterations
-        over the last m_scf differences."""
-        pp = self.pp
+        iteration the SQ density, energy, Poisson and Vxc, then the Veff
+        update.  mixing="pulay": the reference's Veff_old + beta_scf res with
+        an Anderson step every p_scf iterations over the last m_scf
+        differences.  mixing="response": the Newton step of response_step
+        (no counterpart in the reference)."""
+        if mixing not in ("pulay", "response"):
+            raise ValueError(f"unknown mixing {mixing!r}")
+        pp = dict(self.pp)
+        if scf_tol is not None:
+            pp["scf_tol"] = scf_tol
         m, p, beta = pp["m_scf"], pp["p_scf"], pp["beta_scf"]
         if self.veff is None:
             self.initial_potential()
@@ -156,24 +216,34 @@ class SCF:
         veff_old = self.veff.copy()
         history = []
         err, count, last = pp["scf_tol"] + 1.0, 1, None
+        n_prod = 0
         while err > pp["scf_tol"] and count <= pp["scf_maxiter"]:
             last = self.iteration()
             err = last["scf_err"]
             res = last["veff_new"] - veff_old
-            vnew = veff_old + beta * res
-            if count > 1:
-                k = (count - 2) % m
-                DX[k] = veff_old - Xold
-                DF[k] = res - Fold
-            Xold, Fold = veff_old.copy(), res.copy()
-            if count % p == 0 and count > 1:
-                vnew = vnew - anderson_extrapolation(DX, DF, res, beta)
+            gres = None
+            if mixing == "response":
+                if err > pp["scf_tol"]:
+                    x, gres = self.response_step(res, last["rho"], krylov, products)
+                    n_prod += krylov
+                    vnew = veff_old + x
+                else:
+                    vnew = veff_old
+            else:
+                vnew = veff_old + beta * res
+                if count > 1:
+                    k = (count - 2) % m
+                    DX[k] = veff_old - Xold
+                    DF[k] = res - Fold
+                Xold, Fold = veff_old.copy(), res.copy()
+                if count % p == 0 and count > 1:
+                    vnew = vnew - anderson_extrapolation(DX, DF, res, beta)
             veff_old = vnew
             if err > pp["scf_tol"] and count < pp["scf_maxiter"]:
                 self.veff = vnew
             history.append({"lambda_f": last["lambda_f"], "etot": last["etot"], "scf_err": err,
                             "poisson": last["poisson"], "fermi_iters": last["stats"]["fermi_iters"],
-                            "t_sq": last["t_sq"], "t_rest": last["t_rest"]})
+                            "t_sq": last["t_sq"], "t_rest": last["t_rest"], "gmres_res": gres})
             count += 1
         return {"iterations": count - 1, "converged": err <= pp["scf_tol"], "free_energy": last["etot"],
-                "lambda_f": last["lambda_f"], "history": history}
+                "lambda_f": last["lambda_f"], "history": history, "response_products": n_prod}

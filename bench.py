@@ -222,6 +222,9 @@ def main():
         log("bench: scf iterations")
         from sqdft_amd.scf import SCF
         scf = SCF(S, precision=args.precision, batch_nodes=args.batch_nodes)
+        t0 = time.perf_counter()
+        scf.Pseudocharge()
+        t_pc = time.perf_counter() - t0
         scf.initial_potential()
         it1 = scf.iteration()
         scf.veff = it1["veff_new"]
@@ -237,7 +240,7 @@ def main():
         out["scf_step"] = {"s": it2["t_sq"] + it2["t_rest"], "sq_s": it2["t_sq"],
                            "energy_poisson_vxc_s": it2["t_rest"], "poisson_iters": it2["poisson"][0],
                            "scf_err": it2["scf_err"], "lambda_f": it2["lambda_f"],
-                           "forces_local_s": t_loc, "forces_nonlocal_s": t_nl,
+                           "forces_local_s": t_loc, "forces_nonlocal_s": t_nl, "pseudocharge_s": t_pc,
                            "what": "second SCF iteration from the atomic starting density, unmixed Veff; "
                                    "then local and nonlocal forces at that state"}
     if not args.no_scf:

@@ -195,6 +195,8 @@ typedef struct sqg_local_spec {
     const int *img_type, *img_atom;
     const int *img_box;           /* 6 per replica: intersection lo x,y,z, hi x,y,z */
     int n_atm;
+    const double *const *uu;      /* per type: atomic valence density uu(r) (sqg_pseudocharge only) */
+    const double *const *duu;     /* per type: its spline derivative table (sqg_pseudocharge only) */
 } sqg_local_spec;
 
 /* Local forces <- ForcesLocal(DS_SQ*) (src/forces.cpp:20-315, inc/func_sq.h:534)
@@ -205,6 +207,15 @@ typedef struct sqg_local_spec {
  * inconsistent spec, SQG_E_SPLINE when a distance leaves a radial grid. */
 int sqg_forces_local(sqg_ctx *ctx, const sqg_local_spec *spec, const double *phi, const double *b,
                      const double *b_tilda, const double *vc, double *floc, double *fcorr);
+
+/* Pseudocharge and starting density <- ChargeDensity(DS_SQ*)
+ * (src/initialize.cpp:471-784, first MD step, one rank; inc/func_sq.h:
+ * ChargeDensity) over the replicas of spec (uu, duu required): b, rho =
+ * rho_at scaled to -Ncharge electrons, b_tilda (scaled to Ncharge) and vc
+ * (shifted) as n^3 arrays, and ncharge, eself, ecorr -- the outputs of
+ * sqdft_system_pseudocharge, computed on the device; any output may be NULL. */
+int sqg_pseudocharge(sqg_ctx *ctx, const sqg_local_spec *spec, double *b, double *rho, double *ncharge,
+                     double *eself, double *ecorr, double *b_tilda, double *vc);
 
 #ifdef __cplusplus
 }

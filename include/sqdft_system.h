@@ -130,6 +130,9 @@ int  sqdft_system_rb_images(const sqdft_system *S, int *n_img, double *xyz, int 
 /* Spline derivative table of type t's r V_loc(r) (getYD_gen, src/spline.cpp:57-126),
  * `size` doubles: with sqdft_system_radial's r and rVloc it defines V_loc. */
 int  sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d);
+/* The same for the atomic valence density uu(r) (ChargeDensity's Duu,
+ * src/initialize.cpp:511-512). */
+int  sqdft_system_uu_deriv(const sqdft_system *S, int t, double *d);
 
 /* Poisson-solver and SCF parameters (".input" poisson_tol, poisson_maxiter,
  * beta_aaj, m_aaj, p_aaj, scf_tol, scf_miniter, scf_maxiter, beta_scf,

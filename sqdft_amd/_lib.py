@@ -30,7 +30,8 @@ class ScfParams(C.Structure):
 class LocalSpec(C.Structure):
     _fields_ = [("n_typ", C.c_int), ("size", c_int_p), ("r", C.POINTER(c_dbl_p)), ("rvloc", C.POINTER(c_dbl_p)),
                 ("dvloc", C.POINTER(c_dbl_p)), ("Z", c_dbl_p), ("n_img", C.c_int), ("img_xyz", c_dbl_p),
-                ("img_type", c_int_p), ("img_atom", c_int_p), ("img_box", c_int_p), ("n_atm", C.c_int)]
+                ("img_type", c_int_p), ("img_atom", c_int_p), ("img_box", c_int_p), ("n_atm", C.c_int),
+                ("uu", C.POINTER(c_dbl_p)), ("duu", C.POINTER(c_dbl_p))]
 
 
 class Spec(C.Structure):
@@ -96,6 +97,9 @@ SIGNATURES = {
                                             c_dbl_p]),
     "sqg_forces_local": (C.c_int, [C.c_void_p, C.POINTER(LocalSpec), c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
                                    c_dbl_p]),
+    "sqg_pseudocharge": (C.c_int, [C.c_void_p, C.POINTER(LocalSpec), c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
+                                   c_dbl_p, c_dbl_p]),
+    "sqdft_system_uu_deriv": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
     "sqdft_system_rb_images": (C.c_int, [C.c_void_p, c_int_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
     "sqdft_system_vloc_deriv": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
     "sqdft_system_scf_params": (C.c_int, [C.c_void_p, C.c_void_p]),

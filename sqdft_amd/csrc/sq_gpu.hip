@@ -88,7 +88,7 @@ struct sqg_ctx {
     // AAJ Poisson: iterates, residual, history, rhs, result, block partials
     DBuf pz_phi, pz_new, pz_res, pz_X, pz_F, pz_rhs, pz_out, pz_DX, pz_DF, pz_part;
     // local forces: replica boxes, radial tables, block partials, error flag
-    DBuf lf_V, lf_Vt, lf_bJ, lf_bJt, lf_img, lf_R, lf_RV, lf_DV, lf_toff, lf_tsize, lf_Z, lf_part, lf_err;
+    DBuf lf_V, lf_Vt, lf_bJ, lf_bJt, lf_img, lf_R, lf_RV, lf_DV, lf_RU, lf_DU, lf_toff, lf_tsize, lf_Z, lf_part, lf_err;
     bool have_state = false;
     double lambda_f = 0.0;
     // device timing of the hot kernels over the last density call (which:
@@ -1423,32 +1423,49 @@ int sqg_total_energy(sqg_ctx *c, const double *phi, const double *b, const doubl
     return SQG_OK;
 }
 
-int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, const double *b, const double *bt,
-                     const double *vc, double *floc, double *fcorr)
+}  // extern "C"
+
+namespace {
+
+// Replica boxes of a local-pseudopotential pass (ForcesLocal / ChargeDensity):
+// validated spec, radial tables on the device, and batches of replicas whose
+// boxes (intersection + 2 FDn per side) fit the scratch arrays.
+struct LocPlan {
+    std::vector<sqk::LImg> img;
+    std::vector<long> npts;
+    long cap = 0, bmax = 1;
+    sqk::LocArgs A{};
+};
+
+int loc_prepare(sqg_ctx *c, const sqg_local_spec *L, bool with_uu, LocPlan &P)
 {
-    if (!c || !L || !phi || !b || !bt || !vc || !floc || !fcorr) return SQG_E_ARG;
     const int n = c->p.n, F = c->p.FDn;
     if (L->n_typ < 1 || L->n_typ > SQDFT_MAX_TYPES || L->n_img < 0 || L->n_atm < 1 || !L->size || !L->r ||
-        !L->rvloc || !L->dvloc || !L->Z || (L->n_img > 0 && (!L->img_xyz || !L->img_type || !L->img_atom || !L->img_box)))
+        !L->rvloc || !L->dvloc || !L->Z || (with_uu && (!L->uu || !L->duu)) ||
+        (L->n_img > 0 && (!L->img_xyz || !L->img_type || !L->img_atom || !L->img_box)))
         return SQG_E_ARG;
-    HIPCHK(hipSetDevice(c->p.device));
     // radial tables of all types side by side
     std::vector<int> toff(L->n_typ), tsize(L->n_typ);
-    std::vector<double> R, RV, DV;
+    std::vector<double> R, RV, DV, RU, DU;
     for (int t = 0; t < L->n_typ; ++t) {
         if (L->size[t] < 2 || !L->r[t] || !L->rvloc[t] || !L->dvloc[t]) return SQG_E_ARG;
+        if (with_uu && (!L->uu[t] || !L->duu[t])) return SQG_E_ARG;
         toff[t] = (int)R.size();
         tsize[t] = L->size[t];
         R.insert(R.end(), L->r[t], L->r[t] + L->size[t]);
         RV.insert(RV.end(), L->rvloc[t], L->rvloc[t] + L->size[t]);
         DV.insert(DV.end(), L->dvloc[t], L->dvloc[t] + L->size[t]);
+        if (with_uu) {
+            RU.insert(RU.end(), L->uu[t], L->uu[t] + L->size[t]);
+            DU.insert(DU.end(), L->duu[t], L->duu[t] + L->size[t]);
+        }
     }
-    // replicas: every box index must lie in [0, n-1] (the kernels gather
-    // phi, b, b~, vc there without wrapping)
-    std::vector<sqk::LImg> img(L->n_img);
-    std::vector<long> npts(L->n_img);
+    // replicas: every box index must lie in [0, n-1] (the kernels address the
+    // n^3 arrays there without wrapping)
+    P.img.assign(L->n_img, sqk::LImg{});
+    P.npts.assign(L->n_img, 0);
     for (int m = 0; m < L->n_img; ++m) {
-        sqk::LImg &I = img[m];
+        sqk::LImg &I = P.img[m];
         I.x = L->img_xyz[3 * m]; I.y = L->img_xyz[3 * m + 1]; I.z = L->img_xyz[3 * m + 2];
         I.type = L->img_type[m]; I.pad = 0;
         if (I.type < 0 || I.type >= L->n_typ || L->img_atom[m] < 0 || L->img_atom[m] >= L->n_atm) return SQG_E_ARG;
@@ -1459,68 +1476,104 @@ int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, con
             if (I.lo[d] < 0 || I.hi[d] > n - 1 || I.lo[d] > I.hi[d]) return SQG_E_ARG;
             np *= I.hi[d] - I.lo[d] + 1 + 4 * F;
         }
-        npts[m] = np;
+        P.npts[m] = np;
+        P.bmax = std::max(P.bmax, np);
     }
-    const size_t N = (size_t)n * n * n;
-    // batches of replicas whose boxes fit the scratch (64M points per array)
-    const long CAP = 1L << 26;
-    long maxb = 1;
-    for (long v : npts) maxb = std::max(maxb, v);
-    const long cap = std::max(CAP, maxb);
-    const int GX = 64;
-    for (DBuf *d : {&c->lf_V, &c->lf_Vt, &c->lf_bJ, &c->lf_bJt}) HIPCHK(d->ensure((size_t)cap * sizeof(double)));
-    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X}) HIPCHK(d->ensure(N * sizeof(double)));
-    HIPCHK(c->lf_R.ensure(R.size() * sizeof(double)));
-    HIPCHK(c->lf_RV.ensure(R.size() * sizeof(double)));
-    HIPCHK(c->lf_DV.ensure(R.size() * sizeof(double)));
+    P.cap = std::max(1L << 26, P.bmax);   // 64M points per scratch array
+    for (DBuf *d : {&c->lf_V, &c->lf_Vt, &c->lf_bJ, &c->lf_bJt}) HIPCHK(d->ensure((size_t)P.cap * sizeof(double)));
+    const size_t nr = R.size();
+    for (DBuf *d : {&c->lf_R, &c->lf_RV, &c->lf_DV, &c->lf_RU, &c->lf_DU}) HIPCHK(d->ensure(nr * sizeof(double)));
     HIPCHK(c->lf_toff.ensure(L->n_typ * sizeof(int)));
     HIPCHK(c->lf_tsize.ensure(L->n_typ * sizeof(int)));
     HIPCHK(c->lf_Z.ensure(L->n_typ * sizeof(double)));
-    HIPCHK(c->lf_img.ensure(std::max<size_t>(img.size(), 1) * sizeof(sqk::LImg)));
+    HIPCHK(c->lf_img.ensure(std::max<size_t>(P.img.size(), 1) * sizeof(sqk::LImg)));
     HIPCHK(c->lf_err.ensure(sizeof(int)));
-    HIPCHK(hipMemcpyAsync(c->lf_R.p, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->lf_RV.p, RV.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->lf_DV.p, DV.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_R.p, R.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_RV.p, RV.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->lf_DV.p, DV.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (with_uu) {
+        HIPCHK(hipMemcpyAsync(c->lf_RU.p, RU.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->lf_DU.p, DU.data(), nr * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
     HIPCHK(hipMemcpyAsync(c->lf_toff.p, toff.data(), L->n_typ * sizeof(int), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->lf_tsize.p, tsize.data(), L->n_typ * sizeof(int), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->lf_Z.p, L->Z, L->n_typ * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemsetAsync(c->lf_err.p, 0, sizeof(int), c->st));
+    sqk::LocArgs &A = P.A;
+    A.n = n; A.F = F; A.h = c->p.h; A.h3 = std::pow(c->p.h, 3);
+    for (int a = 0; a <= F; ++a) { A.lap[a] = c->p.coeff_lap[a]; A.grad[a] = c->p.coeff_grad[a]; }
+    A.R = c->lf_R.as<double>(); A.RV = c->lf_RV.as<double>(); A.DV = c->lf_DV.as<double>();
+    A.RU = c->lf_RU.as<double>(); A.DU = c->lf_DU.as<double>();
+    A.toff = c->lf_toff.as<int>(); A.tsize = c->lf_tsize.as<int>(); A.Z = c->lf_Z.as<double>();
+    A.V = c->lf_V.as<double>(); A.Vt = c->lf_Vt.as<double>(); A.bJ = c->lf_bJ.as<double>();
+    A.bJt = c->lf_bJt.as<double>();
+    A.err = c->lf_err.as<int>();
+    A.img = c->lf_img.as<sqk::LImg>();
+    return SQG_OK;
+}
+
+// Runs the potential and charge kernels batch by batch, then body(m0, nb)
+// with A.img / A.nimg set to the batch (replicas m0 .. m0 + nb - 1).
+template <class Body>
+int loc_batches(sqg_ctx *c, LocPlan &P, Body &&body)
+{
+    const int nimg = (int)P.img.size();
+    for (int m0 = 0; m0 < nimg;) {
+        int m1 = m0;
+        long used = 0, bmax = 1;
+        while (m1 < nimg && m1 - m0 < 65535 && used + P.npts[m1] <= P.cap) {
+            P.img[m1].off = used;
+            used += P.npts[m1];
+            bmax = std::max(bmax, P.npts[m1]);
+            ++m1;
+        }
+        const int nb = m1 - m0;
+        HIPCHK(hipMemcpyAsync(c->lf_img.p, P.img.data() + m0, nb * sizeof(sqk::LImg), hipMemcpyHostToDevice, c->st));
+        P.A.nimg = nb;
+        const int gx = (int)std::min<long>((bmax + sqk::PB - 1) / sqk::PB, 64);
+        hipLaunchKernelGGL(sqk::k_loc_potential, dim3(gx, nb), dim3(sqk::PB), 0, c->st, P.A);
+        hipLaunchKernelGGL(sqk::k_loc_charge, dim3(gx, nb), dim3(sqk::PB), 0, c->st, P.A);
+        HIPCHK(hipGetLastError());
+        if (int st = body(m0, nb)) return st;
+        m0 = m1;
+    }
+    return SQG_OK;
+}
+
+int loc_error(sqg_ctx *c)
+{
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, c->lf_err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return err ? SQG_E_SPLINE : SQG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, const double *b, const double *bt,
+                     const double *vc, double *floc, double *fcorr)
+{
+    if (!c || !L || !phi || !b || !bt || !vc || !floc || !fcorr) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    LocPlan P;
+    if (int st = loc_prepare(c, L, false, P)) return st;
+    const size_t N = (size_t)c->p.n * c->p.n * c->p.n;
+    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X}) HIPCHK(d->ensure(N * sizeof(double)));
     HIPCHK(hipMemcpyAsync(c->pz_phi.p, phi, N * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->pz_new.p, b, N * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->pz_res.p, bt, N * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->pz_X.p, vc, N * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemsetAsync(c->lf_err.p, 0, sizeof(int), c->st));
-
-    sqk::LocArgs A{};
-    A.n = n; A.F = F; A.h = c->p.h; A.h3 = std::pow(c->p.h, 3);
-    for (int a = 0; a <= F; ++a) { A.lap[a] = c->p.coeff_lap[a]; A.grad[a] = c->p.coeff_grad[a]; }
-    A.R = c->lf_R.as<double>(); A.RV = c->lf_RV.as<double>(); A.DV = c->lf_DV.as<double>();
-    A.toff = c->lf_toff.as<int>(); A.tsize = c->lf_tsize.as<int>(); A.Z = c->lf_Z.as<double>();
-    A.V = c->lf_V.as<double>(); A.Vt = c->lf_Vt.as<double>(); A.bJ = c->lf_bJ.as<double>();
-    A.bJt = c->lf_bJt.as<double>();
-    A.phi = c->pz_phi.as<double>(); A.b = c->pz_new.as<double>(); A.bt = c->pz_res.as<double>();
-    A.vc = c->pz_X.as<double>();
-    A.err = c->lf_err.as<int>();
+    P.A.phi = c->pz_phi.as<double>(); P.A.b = c->pz_new.as<double>(); P.A.bt = c->pz_res.as<double>();
+    P.A.vc = c->pz_X.as<double>();
     for (int q = 0; q < 3 * L->n_atm; ++q) floc[q] = fcorr[q] = 0.0;
+    const int GX = 64;
     std::vector<double> part;
-    for (int m0 = 0; m0 < L->n_img;) {
-        int m1 = m0;
-        long used = 0, bmax = 1;
-        while (m1 < L->n_img && m1 - m0 < 65535 && used + npts[m1] <= cap) {
-            img[m1].off = used;
-            used += npts[m1];
-            bmax = std::max(bmax, npts[m1]);
-            ++m1;
-        }
-        const int nb = m1 - m0;
-        HIPCHK(hipMemcpyAsync(c->lf_img.p, img.data() + m0, nb * sizeof(sqk::LImg), hipMemcpyHostToDevice, c->st));
+    int st = loc_batches(c, P, [&](int m0, int nb) -> int {
         HIPCHK(c->lf_part.ensure((size_t)nb * GX * 6 * sizeof(double)));
-        A.img = c->lf_img.as<sqk::LImg>();
-        A.nimg = nb;
-        A.part = c->lf_part.as<double>();
-        const int gx = (int)std::min<long>((bmax + sqk::PB - 1) / sqk::PB, GX);
-        hipLaunchKernelGGL(sqk::k_loc_potential, dim3(gx, nb), dim3(sqk::PB), 0, c->st, A);
-        hipLaunchKernelGGL(sqk::k_loc_charge, dim3(gx, nb), dim3(sqk::PB), 0, c->st, A);
-        hipLaunchKernelGGL(sqk::k_loc_force, dim3(GX, nb), dim3(sqk::PB), 0, c->st, A);
+        P.A.part = c->lf_part.as<double>();
+        hipLaunchKernelGGL(sqk::k_loc_force, dim3(GX, nb), dim3(sqk::PB), 0, c->st, P.A);
         HIPCHK(hipGetLastError());
         part.resize((size_t)nb * GX * 6);
         HIPCHK(hipMemcpyAsync(part.data(), c->lf_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -1536,12 +1589,70 @@ int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, con
                 fcorr[d * L->n_atm + a] += f[3 + d];
             }
         }
-        m0 = m1;
-    }
-    int err = 0;
-    HIPCHK(hipMemcpyAsync(&err, c->lf_err.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+        return SQG_OK;
+    });
+    if (st) return st;
+    return loc_error(c);
+}
+
+int sqg_pseudocharge(sqg_ctx *c, const sqg_local_spec *L, double *b_out, double *rho_out, double *ncharge,
+                     double *eself, double *ecorr, double *bt_out, double *vc_out)
+{
+    if (!c || !L) return SQG_E_ARG;
+    HIPCHK(hipSetDevice(c->p.device));
+    LocPlan P;
+    if (int st = loc_prepare(c, L, true, P)) return st;
+    const int n = c->p.n;
+    const size_t N = (size_t)n * n * n;
+    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X}) HIPCHK(d->ensure(N * sizeof(double)));
+    // b, b~, Vc, rho_at accumulate over the batches in replica order
+    for (DBuf *d : {&c->pz_phi, &c->pz_new, &c->pz_res, &c->pz_X})
+        HIPCHK(hipMemsetAsync(d->p, 0, N * sizeof(double), c->st));
+    P.A.gb = c->pz_phi.as<double>(); P.A.gbt = c->pz_new.as<double>(); P.A.gvc = c->pz_res.as<double>();
+    P.A.grho = c->pz_X.as<double>();
+    const int nbk = (int)std::min<size_t>((N + sqk::PB - 1) / sqk::PB, 1024);
+    HIPCHK(c->lf_part.ensure((size_t)nbk * 4 * sizeof(double)));
+    std::vector<double> part((size_t)nbk * 4);
+    double es = 0, nc = 0, est = 0, nct = 0;
+    int st = loc_batches(c, P, [&](int, int) -> int {
+        hipLaunchKernelGGL(sqk::k_pc_gather, dim3(nbk), dim3(sqk::PB), 0, c->st, P.A, c->lf_part.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(part.data(), c->lf_part.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int q = 0; q < nbk; ++q) {
+            es += part[4 * q]; nc += part[4 * q + 1]; est += part[4 * q + 2]; nct += part[4 * q + 3];
+        }
+        return SQG_OK;
+    });
+    if (st) return st;
+    if ((st = loc_error(c))) return st;
+    std::vector<double> b(N), bt(N), vc(N), ra(N);
+    HIPCHK(hipMemcpyAsync(b.data(), c->pz_phi.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(bt.data(), c->pz_new.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(vc.data(), c->pz_res.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(ra.data(), c->pz_X.p, N * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    return err ? SQG_E_SPLINE : SQG_OK;
+    // scaling, Vc shift and Ecorr in grid order (src/initialize.cpp:699-779)
+    const double h = c->p.h;
+    double Nel = 0;
+    for (size_t g = 0; g < N; ++g) Nel += ra[g] * h * h * h;
+    if (rho_out)
+        for (size_t g = 0; g < N; ++g) rho_out[g] = std::fabs(nc / Nel) * ra[g];
+    for (size_t g = 0; g < N; ++g) bt[g] = std::fabs(nc / nct) * bt[g];
+    // Vcon: V~ - V at node (0,0,0) summed over the replicas holding it, i.e. Vc(0,0,0)
+    const double V000 = vc[0], Vcon = vc[0];
+    double Ec = 0;
+    for (size_t g = 0; g < N; ++g) {
+        vc[g] += -V000 + Vcon;
+        Ec += 0.5 * (bt[g] + b[g]) * vc[g] * std::pow(h, 3);
+    }
+    if (b_out) std::memcpy(b_out, b.data(), N * sizeof(double));
+    if (bt_out) std::memcpy(bt_out, bt.data(), N * sizeof(double));
+    if (vc_out) std::memcpy(vc_out, vc.data(), N * sizeof(double));
+    if (ncharge) *ncharge = nc;
+    if (eself) *eself = es;
+    if (ecorr) *ecorr = Ec + es - est;
+    return SQG_OK;
 }
 
 }  // extern "C"

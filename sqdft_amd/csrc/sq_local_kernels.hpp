@@ -41,6 +41,9 @@ struct LocArgs {
     const double *phi, *b, *bt, *vc;
     double *part;                // [nimg][gridDim.x][6] block partials
     int *err;                    // set to 1 when r leaves a radial grid
+    // pseudocharge gather
+    const double *RU, *DU;       // per type r-grid values of uu and their spline derivatives
+    double *gb, *gbt, *gvc, *grho;  // n^3 accumulators (b, b~, Vc, rho_at)
 };
 
 // Interval of the reference's single-point search (src/spline.cpp:11-55):
@@ -171,6 +174,50 @@ __global__ void __launch_bounds__(PB) k_loc_force(LocArgs A)
     grid_block_sum<6>(acc, red);
     if (threadIdx.x == 0)
         for (int c = 0; c < 6; ++c) A.part[((long)blockIdx.y * gridDim.x + blockIdx.x) * 6 + c] = acc[c];
+}
+
+// Pseudocharge gather (ChargeDensity, src/initialize.cpp:565-630): every grid
+// node adds, replica by replica in list order, b_J, b~_J, V~_J - V_J and the
+// atomic density uu(r) of the replicas whose intersection holds it -- the
+// order the reference accumulates them in -- and the per-node terms of
+// Eself, Ncharge, Eself~, Ncharge~ are reduced per block.
+__global__ void __launch_bounds__(PB) k_pc_gather(LocArgs A, double *part4)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[(PB / 64) * 4];
+    const int n = A.n, F = A.F;
+    const long N = (long)n * n * n;
+    const double h = A.h, h3 = h * h * h;
+    double acc[4] = {0, 0, 0, 0};
+    int bad = 0;
+    for (long g = blockIdx.x * (long)PB + threadIdx.x; g < N; g += (long)gridDim.x * PB) {
+        const int i = (int)(g % n), j = (int)((g / n) % n), k = (int)(g / ((long)n * n));
+        double b = A.gb[g], bt = A.gbt[g], vc = A.gvc[g], ra = A.grho[g];
+        for (int m = 0; m < A.nimg; ++m) {
+            const LImg &I = A.img[m];
+            if (i < I.lo[0] || i > I.hi[0] || j < I.lo[1] || j > I.hi[1] || k < I.lo[2] || k > I.hi[2]) continue;
+            int nx, ny, nz;
+            box_dims(I, 2 * F, nx, ny, nz);
+            const long l = I.off + ((long)(k - I.lo[2] + 2 * F) * ny + (j - I.lo[1] + 2 * F)) * nx + (i - I.lo[0] + 2 * F);
+            const double bj = A.bJ[l], btj = A.bJt[l], v = A.V[l], vt = A.Vt[l];
+            b += bj;
+            acc[0] += 0.5 * bj * v * h3;
+            acc[1] += bj * h3;
+            bt += btj;
+            acc[2] += 0.5 * btj * vt * h3;
+            acc[3] += btj * h3;
+            vc += vt - v;
+            const double dx = i * h - I.x, dy = j * h - I.y, dz = k * h - I.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const int t = I.type;
+            ra += spline_at(A.R + A.toff[t], A.RU + A.toff[t], A.DU + A.toff[t], A.tsize[t], r, &bad);
+        }
+        A.gb[g] = b; A.gbt[g] = bt; A.gvc[g] = vc; A.grho[g] = ra;
+    }
+    if (bad) *A.err = 1;
+    grid_block_sum<4>(acc, red);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; ++c) part4[(long)blockIdx.x * 4 + c] = acc[c];
 }
 
 }  // namespace sqk

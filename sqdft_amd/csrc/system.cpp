@@ -852,4 +852,12 @@ int sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d)
     return SQDFT_OK;
 }
 
+int sqdft_system_uu_deriv(const sqdft_system *S, int t, double *d)
+{
+    if (!S || !d || t < 0 || t >= S->info.n_typ) return SQDFT_E_ARG;
+    const PsdTable &P = S->psd[t];
+    spline_derivs(P.r.data(), P.uu.data(), d, P.size);
+    return SQDFT_OK;
+}
+
 }  // extern "C"

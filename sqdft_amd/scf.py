@@ -66,11 +66,16 @@ def anderson_extrapolation(DX: np.ndarray, DF: np.ndarray, res: np.ndarray, beta
 class SCF:
     """Owns a SpectralQuadrature context and the SCF state of one geometry."""
 
-    def __init__(self, system: System, precision: str = "fp64", **sq_kw):
+    def __init__(self, system: System, precision: str = "fp64", pseudocharge: str = "device", **sq_kw):
+        """pseudocharge: "device" (sqg_pseudocharge) or "host"
+        (sqdft_system_pseudocharge); the two agree to rounding."""
         self.S = system
         self.sq = SpectralQuadrature(system, precision=precision, **sq_kw)
         self._lib = self.sq._lib
-        pc = system.pseudocharge()
+        self._spec = None
+        if pseudocharge not in ("device", "host"):
+            raise ValueError(f"unknown pseudocharge {pseudocharge!r}")
+        pc = self.Pseudocharge() if pseudocharge == "device" else system.pseudocharge()
         self.b, self.rho = pc["b"], pc["rho"]
         self.ncharge, self.eself, self.ecorr = pc["ncharge"], pc["eself"], pc["ecorr"]
         self.b_tilda, self.vc = pc["b_tilda"], pc["vc"]
@@ -113,26 +118,48 @@ class SCF:
             self.sq._ctx, *[L.dptr(a) for a in arrs], ebs, eent, self.eself, self.ecorr, self.S.n_atm, C.byref(e)))
         return e.value
 
+    def _local_spec(self):
+        """The replicas and radial tables of the local pseudopotential passes
+        (built once per geometry; the arrays stay referenced by the spec)."""
+        if self._spec is None:
+            S, nt = self.S, self.S.info["n_typ"]
+            tabs = [S.vloc_spline(t) for t in range(nt)]
+            keys = ("r", "rVloc", "d", "uu", "duu")
+            keep = [np.ascontiguousarray(tb[k], dtype=np.float64) for tb in tabs for k in keys]
+            ptrs = lambda k: (L.c_dbl_p * nt)(*[L.dptr(keep[len(keys) * t + k]) for t in range(nt)])  # noqa: E731
+            size = np.array([tb["r"].size for tb in tabs], dtype=np.int32)
+            Z = np.array([float(S.type_info(t)["Z"]) for t in range(nt)])
+            im = S.rb_images()
+            xyz = np.ascontiguousarray(im["xyz"], dtype=np.float64).reshape(-1)
+            typ, atm = np.ascontiguousarray(im["type"]), np.ascontiguousarray(im["atom"])
+            box = np.ascontiguousarray(im["box"], dtype=np.int32).reshape(-1)
+            arrs = (ptrs(0), ptrs(1), ptrs(2), ptrs(3), ptrs(4))
+            spec = L.LocalSpec(n_typ=nt, size=L.iptr(size), r=arrs[0], rvloc=arrs[1], dvloc=arrs[2], Z=L.dptr(Z),
+                               n_img=typ.size, img_xyz=L.dptr(xyz), img_type=L.iptr(typ), img_atom=L.iptr(atm),
+                               img_box=L.iptr(box), n_atm=S.n_atm, uu=arrs[3], duu=arrs[4])
+            self._spec = (spec, keep, arrs, size, Z, xyz, typ, atm, box)
+        return self._spec[0]
+
+    def Pseudocharge(self) -> dict:
+        """ChargeDensity on the device (sqg_pseudocharge): b, rho (rho_at
+        scaled), ncharge, eself, ecorr, b_tilda, vc."""
+        n3 = self.S.n ** 3
+        b, rho, bt, vc = np.zeros(n3), np.zeros(n3), np.zeros(n3), np.zeros(n3)
+        nc, es, ec = C.c_double(), C.c_double(), C.c_double()
+        L.check("sqg_pseudocharge", self._lib.sqg_pseudocharge(
+            self.sq._ctx, C.byref(self._local_spec()), L.dptr(b), L.dptr(rho), C.byref(nc), C.byref(es),
+            C.byref(ec), L.dptr(bt), L.dptr(vc)))
+        return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value,
+                "b_tilda": bt, "vc": vc}
+
     def ForcesLocal(self) -> tuple[np.ndarray, np.ndarray]:
         """Local forces and their correction term, each (n_atm, 3), from the
         current phi and the pseudocharge (ForcesLocal, src/forces.cpp:20-315)."""
-        S, nt, na = self.S, self.S.info["n_typ"], self.S.n_atm
-        tabs = [S.vloc_spline(t) for t in range(nt)]
-        keep = [np.ascontiguousarray(x, dtype=np.float64) for tb in tabs for x in (tb["r"], tb["rVloc"], tb["d"])]
-        ptrs = lambda k: (L.c_dbl_p * nt)(*[L.dptr(keep[3 * t + k]) for t in range(nt)])  # noqa: E731
-        size = np.array([tb["r"].size for tb in tabs], dtype=np.int32)
-        Z = np.array([float(S.type_info(t)["Z"]) for t in range(nt)])
-        im = S.rb_images()
-        xyz = np.ascontiguousarray(im["xyz"], dtype=np.float64).reshape(-1)
-        typ, atm = np.ascontiguousarray(im["type"]), np.ascontiguousarray(im["atom"])
-        box = np.ascontiguousarray(im["box"], dtype=np.int32).reshape(-1)
-        spec = L.LocalSpec(n_typ=nt, size=L.iptr(size), r=ptrs(0), rvloc=ptrs(1), dvloc=ptrs(2), Z=L.dptr(Z),
-                           n_img=typ.size, img_xyz=L.dptr(xyz), img_type=L.iptr(typ), img_atom=L.iptr(atm),
-                           img_box=L.iptr(box), n_atm=na)
+        na = self.S.n_atm
         arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (self.phi, self.b, self.b_tilda, self.vc)]
         fl, fc = np.zeros(3 * na), np.zeros(3 * na)
-        L.check("sqg_forces_local", self._lib.sqg_forces_local(self.sq._ctx, C.byref(spec), *[L.dptr(a) for a in arrs],
-                                                               L.dptr(fl), L.dptr(fc)))
+        L.check("sqg_forces_local", self._lib.sqg_forces_local(self.sq._ctx, C.byref(self._local_spec()),
+                                                               *[L.dptr(a) for a in arrs], L.dptr(fl), L.dptr(fc)))
         return fl.reshape(3, na).T.copy(), fc.reshape(3, na).T.copy()
 
     def ForcesTotal(self) -> np.ndarray:

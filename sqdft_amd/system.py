@@ -166,11 +166,12 @@ class System:
         return {"xyz": xyz.reshape(m, 3), "type": typ, "atom": atm, "box": box.reshape(m, 6)}
 
     def vloc_spline(self, t: int) -> dict:
-        """r, r V_loc(r) and its spline derivative table for type t."""
+        """r, r V_loc(r), uu(r) and their spline derivative tables for type t."""
         rad = self.radial(t)
-        d = np.zeros(rad["r"].size)
+        d, du = np.zeros(rad["r"].size), np.zeros(rad["r"].size)
         L.check("sqdft_system_vloc_deriv", self._lib.sqdft_system_vloc_deriv(self._h, t, L.dptr(d)))
-        return {"r": rad["r"], "rVloc": rad["rVloc"], "d": d}
+        L.check("sqdft_system_uu_deriv", self._lib.sqdft_system_uu_deriv(self._h, t, L.dptr(du)))
+        return {"r": rad["r"], "rVloc": rad["rVloc"], "d": d, "uu": rad["uu"], "duu": du}
 
     def poisson_params(self) -> dict:
         """AAJ and SCF parameters of the input: tol, maxiter, beta, m, p of the

@@ -259,6 +259,16 @@ def main():
                          "response_products": r["response_products"], "s": time.perf_counter() - t0,
                          "free_energy": r["free_energy"]}
             sc.close()
+        log("bench: NVE steps on the Al test input")
+        from sqdft_amd.md import NVE
+        sc = SCF(A)
+        dyn = NVE(sc, masses=[26.9815385], time_step=1.0)
+        recs = dyn.run(4)
+        sc.close()
+        out["md"] = {"workload": "reference tests/Al input, NVE 1 fs, velocities at the electronic T, "
+                                 "charge extrapolation, scf_tol 1e-4",
+                     "s_per_step": [r["seconds"] for r in recs], "scf_iterations": [r["scf_iterations"] for r in recs],
+                     "TE": [r["TE"] for r in recs]}
         A.close()
         out["scf_mixing"] = {"workload": "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6",
                              **mix}

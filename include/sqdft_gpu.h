@@ -213,9 +213,11 @@ int sqg_forces_local(sqg_ctx *ctx, const sqg_local_spec *spec, const double *phi
  * ChargeDensity) over the replicas of spec (uu, duu required): b, rho =
  * rho_at scaled to -Ncharge electrons, b_tilda (scaled to Ncharge) and vc
  * (shifted) as n^3 arrays, and ncharge, eself, ecorr -- the outputs of
- * sqdft_system_pseudocharge, computed on the device; any output may be NULL. */
+ * sqdft_system_pseudocharge, computed on the device, and rho_at, the unscaled
+ * superposition of the atomic densities (what ChargeExtrapolation_MD
+ * subtracts, src/md.cpp:185); any output may be NULL. */
 int sqg_pseudocharge(sqg_ctx *ctx, const sqg_local_spec *spec, double *b, double *rho, double *ncharge,
-                     double *eself, double *ecorr, double *b_tilda, double *vc);
+                     double *eself, double *ecorr, double *b_tilda, double *vc, double *rho_at);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,10 @@ int  sqdft_system_pseudocharge(const sqdft_system *S, double *b, double *rho, do
  * lo x,y,z then hi x,y,z (inclusive node indices). */
 int  sqdft_system_rb_images(const sqdft_system *S, int *n_img, double *xyz, int *type, int *atom,
                             int *box);
+/* Moves the atoms (xyz[3*n_atm], type order, as sqdft_system_atoms) and
+ * rebuilds the images and projector tables for the new positions (what
+ * Replica_atoms_Rcut + NonlocalProjectors redo every MD step, src/scf.cpp:349-350). */
+int  sqdft_system_set_positions(sqdft_system *S, const double *xyz);
 /* Spline derivative table of type t's r V_loc(r) (getYD_gen, src/spline.cpp:57-126),
  * `size` doubles: with sqdft_system_radial's r and rVloc it defines V_loc. */
 int  sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d);

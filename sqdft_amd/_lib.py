@@ -98,8 +98,9 @@ SIGNATURES = {
     "sqg_forces_local": (C.c_int, [C.c_void_p, C.POINTER(LocalSpec), c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
                                    c_dbl_p]),
     "sqg_pseudocharge": (C.c_int, [C.c_void_p, C.POINTER(LocalSpec), c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p,
-                                   c_dbl_p, c_dbl_p]),
+                                   c_dbl_p, c_dbl_p, c_dbl_p]),
     "sqdft_system_uu_deriv": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
+    "sqdft_system_set_positions": (C.c_int, [C.c_void_p, c_dbl_p]),
     "sqdft_system_rb_images": (C.c_int, [C.c_void_p, c_int_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
     "sqdft_system_vloc_deriv": (C.c_int, [C.c_void_p, C.c_int, c_dbl_p]),
     "sqdft_system_scf_params": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -1596,7 +1596,7 @@ int sqg_forces_local(sqg_ctx *c, const sqg_local_spec *L, const double *phi, con
 }
 
 int sqg_pseudocharge(sqg_ctx *c, const sqg_local_spec *L, double *b_out, double *rho_out, double *ncharge,
-                     double *eself, double *ecorr, double *bt_out, double *vc_out)
+                     double *eself, double *ecorr, double *bt_out, double *vc_out, double *rho_at_out)
 {
     if (!c || !L) return SQG_E_ARG;
     HIPCHK(hipSetDevice(c->p.device));
@@ -1649,6 +1649,7 @@ int sqg_pseudocharge(sqg_ctx *c, const sqg_local_spec *L, double *b_out, double 
     if (b_out) std::memcpy(b_out, b.data(), N * sizeof(double));
     if (bt_out) std::memcpy(bt_out, bt.data(), N * sizeof(double));
     if (vc_out) std::memcpy(vc_out, vc.data(), N * sizeof(double));
+    if (rho_at_out) std::memcpy(rho_at_out, ra.data(), N * sizeof(double));
     if (ncharge) *ncharge = nc;
     if (eself) *eself = es;
     if (ecorr) *ecorr = Ec + es - est;

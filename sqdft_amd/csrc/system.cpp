@@ -256,6 +256,8 @@ struct sqdft_system {
 
 namespace {
 
+int build_geometry(sqdft_system *S);
+
 int build(const sqdft_spec *spec, sqdft_system *S)
 {
     if (!spec || spec->n < 1 || spec->L <= 0 || spec->n_typ < 1 || spec->n_typ > SQDFT_MAX_TYPES ||
@@ -329,7 +331,17 @@ int build(const sqdft_spec *spec, sqdft_system *S)
     }
     I.n_atm = na;
 
-    // periodic images whose rc-cube meets [-nloc, n-1+nloc]^3 (initialize.cpp:1003-1196)
+    return build_geometry(S);
+}
+
+// Images whose rc-cube meets [-nloc, n-1+nloc]^3 and their projector tables,
+// from the current atom positions (Replica_atoms_Rcut + NonlocalProjectors).
+int build_geometry(sqdft_system *S)
+{
+    sqdft_info &I = S->info;
+    const int na = I.n_atm;
+    S->img_xyz.clear(); S->img_nn.clear(); S->img_type.clear(); S->img_atom.clear(); S->img_off.clear();
+    // periodic images (initialize.cpp:1003-1196)
     const double L = I.L, h = I.h;
     long off = 0;
     for (int a = 0; a < na; ++a) {
@@ -842,6 +854,13 @@ int sqdft_system_rb_images(const sqdft_system *S, int *n_img, double *xyz, int *
     }
     *n_img = (int)im.size();
     return SQDFT_OK;
+}
+
+int sqdft_system_set_positions(sqdft_system *S, const double *xyz)
+{
+    if (!S || !xyz) return SQDFT_E_ARG;
+    std::memcpy(S->atom_xyz.data(), xyz, S->atom_xyz.size() * sizeof(double));
+    return build_geometry(S);
 }
 
 int sqdft_system_vloc_deriv(const sqdft_system *S, int t, double *d)

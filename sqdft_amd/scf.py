@@ -77,6 +77,7 @@ class SCF:
             raise ValueError(f"unknown pseudocharge {pseudocharge!r}")
         pc = self.Pseudocharge() if pseudocharge == "device" else system.pseudocharge()
         self.b, self.rho = pc["b"], pc["rho"]
+        self.rho_at = pc.get("rho_at")
         self.ncharge, self.eself, self.ecorr = pc["ncharge"], pc["eself"], pc["ecorr"]
         self.b_tilda, self.vc = pc["b_tilda"], pc["vc"]
         self.pp = system.poisson_params()
@@ -144,13 +145,13 @@ class SCF:
         """ChargeDensity on the device (sqg_pseudocharge): b, rho (rho_at
         scaled), ncharge, eself, ecorr, b_tilda, vc."""
         n3 = self.S.n ** 3
-        b, rho, bt, vc = np.zeros(n3), np.zeros(n3), np.zeros(n3), np.zeros(n3)
+        b, rho, bt, vc, ra = np.zeros(n3), np.zeros(n3), np.zeros(n3), np.zeros(n3), np.zeros(n3)
         nc, es, ec = C.c_double(), C.c_double(), C.c_double()
         L.check("sqg_pseudocharge", self._lib.sqg_pseudocharge(
             self.sq._ctx, C.byref(self._local_spec()), L.dptr(b), L.dptr(rho), C.byref(nc), C.byref(es),
-            C.byref(ec), L.dptr(bt), L.dptr(vc)))
+            C.byref(ec), L.dptr(bt), L.dptr(vc), L.dptr(ra)))
         return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value,
-                "b_tilda": bt, "vc": vc}
+                "b_tilda": bt, "vc": vc, "rho_at": ra}
 
     def ForcesLocal(self) -> tuple[np.ndarray, np.ndarray]:
         """Local forces and their correction term, each (n_atm, 3), from the
@@ -195,9 +196,37 @@ class SCF:
         return gmres(mv, np.asarray(res, dtype=np.float64).reshape(-1), krylov)
 
     # ------------------------------------------------------------ driver
+    def set_positions(self, xyz) -> None:
+        """New atom positions ((n_atm, 3), type order): images and projector
+        tables on the host and the device, and the pseudocharge (what
+        SQDFT_forces redoes per MD step, src/scf.cpp:346-350).  The starting
+        density becomes rho_at scaled to Ncharge electrons until
+        set_density_guess says otherwise."""
+        self.S.set_positions(xyz)
+        self.sq.set_geometry()
+        self._spec = None
+        pc = self.Pseudocharge()
+        self.b, self.b_tilda, self.vc, self.rho_at = pc["b"], pc["b_tilda"], pc["vc"], pc["rho_at"]
+        self.ncharge, self.eself, self.ecorr = pc["ncharge"], pc["eself"], pc["ecorr"]
+        self.rho = pc["rho"]
+        self.veff = None
+        self.scf_iter = 0
+
+    def set_density_guess(self, rho, keep_phi: bool = True) -> None:
+        """Starting density: rho scaled to hold -Ncharge electrons
+        (src/initialize.cpp:699-735); keep_phi: the next Poisson solve starts
+        from the last phi instead of phi = 1 (src/initialize.cpp:764-765)."""
+        r = np.asarray(rho, dtype=np.float64).reshape(-1)
+        nel = float(np.sum(r)) * self.S.info["h"] ** 3
+        self.rho = abs(self.ncharge / nel) * r
+        if not keep_phi:
+            self.phi = None
+
     def initial_potential(self) -> np.ndarray:
-        """phi from rho_at + b with phi_guess = 1, Vxc(rho_at), Veff (src/scf.cpp:353-372)."""
-        self.phi = self.PoissonSolver_AAJ(self.b + self.rho, np.ones(self.n3))
+        """phi from rho + b (phi_guess = 1, or the last phi after
+        set_density_guess(keep_phi=True)), Vxc(rho), Veff (src/scf.cpp:353-372)."""
+        guess = np.ones(self.n3) if self.phi is None else self.phi
+        self.phi = self.PoissonSolver_AAJ(self.b + self.rho, guess)
         self.vxc = self.ExchangeCorrelationPotential(self.rho)
         self.veff = self.phi + self.vxc
         return self.veff

@@ -58,14 +58,18 @@ class SpectralQuadrature:
         p.device = device
         self._ctx = C.c_void_p()
         L.check("sqg_create", self._lib.sqg_create(C.byref(p), C.byref(self._ctx)))
-        g = system.geometry_arrays()
-        self._geom = g  # kept alive for the call only; the context copies it
+        self.set_geometry()
+        self.lambda_f = 0.0
+        self.last_status = 0
+
+    def set_geometry(self) -> None:
+        """Uploads the system's current images and projector tables (after
+        System.set_positions); the next density step starts afresh."""
+        g = self.system.geometry_arrays()
         L.check("sqg_set_geometry", self._lib.sqg_set_geometry(
             self._ctx, len(g["nlocc"]), L.iptr(g["nlocc"]), L.iptr(g["nproj"]), L.dptr(g["denom"]),
             int(g["n_atm"]), len(g["img_type"]), L.iptr(g["img_type"]), L.iptr(g["img_atom"]),
             L.iptr(g["img_nn"]), L.dptr(g["udv"])))
-        self.lambda_f = 0.0
-        self.last_status = 0
 
     def close(self) -> None:
         if self._ctx:

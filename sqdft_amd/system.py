@@ -151,6 +151,17 @@ class System:
         return {"b": b, "rho": rho, "ncharge": nc.value, "eself": es.value, "ecorr": ec.value,
                 "b_tilda": bt, "vc": vc}
 
+    def set_positions(self, xyz) -> None:
+        """Moves the atoms ((n_atm, 3), type order) and rebuilds the images and
+        projector tables."""
+        x = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1))
+        if x.size != 3 * self.n_atm:
+            raise ValueError(f"xyz must hold 3 * n_atm = {3 * self.n_atm} values")
+        L.check("sqdft_system_set_positions", self._lib.sqdft_system_set_positions(self._h, L.dptr(x)))
+        info = L.Info()   # image count and table length change with the positions
+        L.check("sqdft_system_info", self._lib.sqdft_system_info(self._h, C.byref(info)))
+        self._info = info
+
     def rb_images(self) -> dict:
         """Replicas whose rb-box meets the domain (Replica_atoms,
         src/initialize.cpp:21-200): xyz, type, atom, box (lo xyz, hi xyz)."""

@@ -97,6 +97,7 @@ struct NL {
     const unsigned char *cnt;     // [E^3]
     const T *u;                   // [K][E^3]
     const int *ip;                // [K][E^3]
+    const unsigned short *ip16;   // [K][E^3] the same ids in 16 bits (n_ip <= 65535), else null
     size_t E3;
     int k_alloc;                  // K rows allocated
     const double *s;              // [B][n_ip] projections of the input vector
@@ -156,6 +157,10 @@ __device__ __forceinline__ int bload_i32(__amdgpu_buffer_rsrc_t r, unsigned off)
 __device__ __forceinline__ int bload_u8(__amdgpu_buffer_rsrc_t r, unsigned off)
 {
     return (int)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+__device__ __forceinline__ int bload_u16(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
 }
 
 // Nonlocal term of B node vectors, NL_b(p) = sum_{J,lm} U_J,lm(p) s_b[J,lm]
@@ -952,7 +957,7 @@ struct NGArgs {
     int B;
 };
 
-template <typename T>
+template <typename T, bool IP16>
 __global__ void __launch_bounds__(512, 4) k_nlapplyg(NGArgs<T> A)
 {
     constexpr unsigned TS = sizeof(T);
@@ -1005,7 +1010,9 @@ __global__ void __launch_bounds__(512, 4) k_nlapplyg(NGArgs<T> A)
     const int E3 = (int)A.nl.E3;
     const int KA = A.nl.k_alloc;
     const __amdgpu_buffer_rsrc_t rEU = brsrc(A.nl.u, (size_t)KA * E3 * TS);
-    const __amdgpu_buffer_rsrc_t rEI = brsrc(A.nl.ip, (size_t)KA * E3 * 4);
+    constexpr unsigned IS = IP16 ? 2u : 4u;   // bytes per ELL id
+    const __amdgpu_buffer_rsrc_t rEI =
+        brsrc(IP16 ? (const void *)A.nl.ip16 : (const void *)A.nl.ip, (size_t)KA * E3 * IS);
     const __amdgpu_buffer_rsrc_t rEC = brsrc(A.nl.cnt, (size_t)E3);
     T *out = A.nlv + (size_t)b0 * M3;
     // one extended-grid point: sum its entries against every member's
@@ -1026,7 +1033,8 @@ __global__ void __launch_bounds__(512, 4) k_nlapplyg(NGArgs<T> A)
                 const int j = j0 + c;
                 const bool ok = j < cn;
                 u[c] = bload<T>(rEU, ok ? (unsigned)(j * E3 + g) * TS : OOB);
-                ip[c] = bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * 4u : OOB);
+                ip[c] = IP16 ? bload_u16(rEI, ok ? (unsigned)(j * E3 + g) * IS : OOB)
+                             : bload_i32(rEI, ok ? (unsigned)(j * E3 + g) * IS : OOB);
             }
 #pragma unroll
             for (int c = 0; c < CH; ++c) {

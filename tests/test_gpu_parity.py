@@ -1,6 +1,7 @@
 """GPU parity: the HIP path (through the C-ABI) against the FP64 oracle on the
 same inputs.  FP64 path: 1e-10 relative or better; mixed path: stated
 tolerances below."""
+import os
 import sys
 
 import numpy as np
@@ -154,3 +155,36 @@ def test_electron_count_at_config0_geometry():
         assert g["iters"][i] == r["iters"]
         assert g["chi"][i] == pytest.approx(r["chi"], rel=FP64_TOL)
         assert H.rel(g["mu"][i], r["mu"]) < FP64_TOL
+
+
+@pytest.mark.parametrize("name", ["al32", "lih216"])
+def test_full_size_density_matches_oracle_fixture(name):
+    """One SQ density step over every node of the 32-atom Al cell (32^3
+    nodes, 35^3 cubes, npl 48) and the 216-atom LiH cell (48^3 nodes, 31^3
+    cubes, npl 64) against the FP64 oracle's run on the same inputs
+    (tests/golden/sq_<name>.npz, written by make_sq_fixtures.py): per-node
+    rho, Lanczos counts, Fermi level, Ebs, Eent; mixed precision within its
+    stated 5e-6 (Fermi tolerance 1e-6 on both sides)."""
+    fx = np.load(os.path.join(H.GOLDEN, f"sq_{name}.npz"))
+    S = configs.build(name, seed=0)
+    veff = smooth_veff(S)
+    say(f"{name}: full density step vs oracle fixture")
+    sq = SpectralQuadrature(S, precision="fp64")
+    out = sq.density(veff, scf_iter=1)
+    it = sq.node_data()["iters"]
+    sq.close()
+    err = H.rel(out["rho"], fx["rho"])
+    say(name, "fp64 rho rel err", err, "dlf", out["lambda_f"] - float(fx["lambda_f"]))
+    assert err < FP64_TOL
+    assert np.array_equal(it, fx["iters"].astype(it.dtype))
+    assert out["stats"]["fermi_iters"] == int(fx["fermi_iters"])
+    assert out["lambda_f"] == pytest.approx(float(fx["lambda_f"]), abs=1e-10)
+    assert out["ebs"] == pytest.approx(float(fx["ebs"]), rel=FP64_TOL)
+    assert out["eent"] == pytest.approx(float(fx["eent"]), rel=FP64_TOL)
+    sqm = SpectralQuadrature(S, precision="mixed")
+    mo = sqm.density(veff, scf_iter=1)
+    sqm.close()
+    errm = H.rel(mo["rho"], fx["rho"])
+    say(name, "mixed rho rel err", errm)
+    assert errm < 5e-6
+    assert mo["lambda_f"] == pytest.approx(float(fx["lambda_f"]), abs=2e-6)

@@ -212,6 +212,22 @@ int choose_batch(sqg_ctx *c, int want, int nvec = 2)
     return std::max(B, 1);
 }
 
+// Nodes per batch of a derivative pass (adjoint, tangent) whose extra
+// vectors, ext bytes per node, live in c->adj on top of the batch buffers the
+// density step sized: at most that batch, and what free memory plus adj's
+// current allocation (freed before it grows) holds.
+int choose_batch_adj(sqg_ctx *c, int want, size_t ext)
+{
+    int B = choose_batch(c, want);
+    if (c->Bcap > 0) B = std::min(B, c->Bcap);
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
+        const size_t cap = (size_t)(0.8 * (double)(freeb + c->adj.bytes)) / std::max<size_t>(ext, 1);
+        if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
+    }
+    return std::max(B, 1);
+}
+
 template <typename T>
 ZArgs<T> zargs(sqg_ctx *c, const int *dn, const int *dd)
 {
@@ -1333,7 +1349,7 @@ int sqg_adjoint_veff(sqg_ctx *c, const double *w, int fixed_n, double *dLdV)
     HIPCHK(hipMemsetAsync(c->adj_out.p, 0, (size_t)nn * sizeof(double), c->st));
     const AdjPlan P = adj_plan(c);
     const int plane = n * n;
-    const int cap = choose_batch(c, nn, P.nvec + 2);
+    const int cap = choose_batch_adj(c, nn, (size_t)P.nvec * c->M3 * c->real_size());
     int nz = std::max(1, std::min(n, cap / plane));
     nz = std::min(nz, std::max(1, 65535 / plane));
     for (int z0 = 0; z0 < n; z0 += nz) {
@@ -1357,7 +1373,9 @@ int sqg_tangent_veff(sqg_ctx *c, const double *dV, int fixed_n, double *drho)
     HIPCHK(c->adj_in.ensure((size_t)nn * sizeof(double)));
     HIPCHK(c->adj_out.ensure((size_t)nn * sizeof(double)));
     HIPCHK(hipMemcpyAsync(c->adj_in.p, dV, (size_t)nn * sizeof(double), hipMemcpyHostToDevice, c->st));
-    const int cap = std::min(choose_batch(c, nn, 8), 65535);
+    const int cap = std::min(choose_batch_adj(c, nn, 6 * (size_t)c->M3 * c->real_size() +
+                                                     (size_t)(c->p.npl / 2 + 1) * 3 * sizeof(double)),
+                             65535);
     for (int s0 = 0; s0 < nn; s0 += cap) {
         const int B = std::min(cap, nn - s0);
         st = c->p.precision == SQG_MIXED ? tangent_batch<float>(c, s0, B) : tangent_batch<double>(c, s0, B);

@@ -203,9 +203,11 @@ def main():
         log("bench: mixed vs fp64 accuracy at fermi_tol 1e-12")
         r64 = SpectralQuadrature(S, precision="fp64", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
         t64 = r64.density(veff, scf_iter=1)
+        f64 = r64.ForcesNonLocal()
         r64.close()
         r32 = SpectralQuadrature(S, precision="mixed", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
         t32 = r32.density(veff, scf_iter=1)
+        f32 = r32.ForcesNonLocal()
         r32.close()
         err_t = float(np.max(np.abs(t32["rho"] - t64["rho"])) / np.max(np.abs(t64["rho"])))
         out["mixed"] = {"value": nn / tm, "unit": "nodes/s", "ms_per_step": 1e3 * tm,
@@ -213,7 +215,9 @@ def main():
                         "rho_rel_err_vs_fp64": err_t,
                         "lambda_f_err": abs(t32["lambda_f"] - t64["lambda_f"]),
                         "rho_rel_err_at_fermi_tol_1e-6": err,
-                        "ebs_rel_err": abs(t32["ebs"] - t64["ebs"]) / abs(t64["ebs"])}
+                        "ebs_rel_err": abs(t32["ebs"] - t64["ebs"]) / abs(t64["ebs"]),
+                        "nonlocal_force_max_err": float(np.max(np.abs(f32 - f64))),
+                        "nonlocal_force_max": float(np.max(np.abs(f64)))}
     sq.close()
     if not args.no_scf:
         # one SCF iteration on the workload's own pseudocharge: SQ density,

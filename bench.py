@@ -118,7 +118,7 @@ def main():
                          "counter runs can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); "
               "the SQ node loop shards by node bricks with one Veff all-gather per SCF step, "
               "which is not run here", file=sys.stderr)
         sys.exit(2)

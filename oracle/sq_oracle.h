@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-/* status codes (mirrors the reference's exit paths, SURVEY-listed) */
+/* status codes (mirror the reference's exit paths) */
 #define SQO_OK 0
 #define SQO_ERR_NPL_NOT_MULT4   -1   /* sq.cpp:523-524 */
 #define SQO_ERR_NEG_RHO         -2   /* sq.cpp:764-765 */

@@ -318,10 +318,10 @@ void nlapply_go(sqg_ctx *c, int B, const NArgs<T> &A)
     hipLaunchKernelGGL(fn, dim3(B), dim3(512), shm, c->st, A);
 }
 
-template <typename T, int MAXP, int FORMED, int SZ, int NT>
+template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU>
 void projs_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
 {
-    auto fn = k_projs<T, MAXP, FORMED, SZ, NT>;
+    auto fn = k_projs<T, MAXP, FORMED, SZ, NT, NPU>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     hipLaunchKernelGGL(fn, grid, dim3(NT), shm, c->st, P);
 }
@@ -340,12 +340,16 @@ bool launch_projs(sqg_ctx *c, dim3 grid, PArgs<T> &P, int mp)
             const size_t shs = sa + (size_t)pu * NG * sizeof(T) * sz;
             if (shs > 160 * 1024) continue;
             P.pu = pu;
+            // projector values are loaded and summed for npu of the mp slots
+            const int npu = mp == 1 ? 1 : c->max_nproj == 3 ? 3 : 4;
             if (sz == D) {
-                if (mp == 1) projs_go<T, 1, FORMED, D, NT>(c, grid, shs, P);
-                else projs_go<T, 4, FORMED, D, NT>(c, grid, shs, P);
+                if (npu == 1) projs_go<T, 1, FORMED, D, NT, 1>(c, grid, shs, P);
+                else if (npu == 3) projs_go<T, 4, FORMED, D, NT, 3>(c, grid, shs, P);
+                else projs_go<T, 4, FORMED, D, NT, 4>(c, grid, shs, P);
             } else {
-                if (mp == 1) projs_go<T, 1, FORMED, H, NT>(c, grid, shs, P);
-                else projs_go<T, 4, FORMED, H, NT>(c, grid, shs, P);
+                if (npu == 1) projs_go<T, 1, FORMED, H, NT, 1>(c, grid, shs, P);
+                else if (npu == 3) projs_go<T, 4, FORMED, H, NT, 3>(c, grid, shs, P);
+                else projs_go<T, 4, FORMED, H, NT, 4>(c, grid, shs, P);
             }
             return true;
         }

@@ -738,12 +738,15 @@ __device__ __forceinline__ int first_cz_at_least(const ImgMeta *im, int cnt, int
     return lo;
 }
 
-template <typename T, int MAXP, int FORMED, int SZ, int NT>
+template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU>
 __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
 {
     constexpr int NV = NG * MAXP;
     constexpr int LV = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : 0;
     static_assert(LV > 0, "k_projs: NG * MAXP must be 8, 16 or 32");
+    // projector values are loaded and summed for the first NPU of the MAXP
+    // slots (3 for a table of p projectors); the other sums stay exact zeros
+    static_assert(NPU >= 1 && NPU <= MAXP, "k_projs: NPU of MAXP slots");
     constexpr unsigned TS = sizeof(T);
     constexpr int RPI = SZ >= 4 ? 2 : 3;   // rounds of 64 box points per U load batch
     constexpr int PPT = SZ >= 4 ? 2 : 3;   // staged points per thread per load batch
@@ -895,7 +898,7 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                 // RPI rounds of 64 box points at a time, their U loads issued together
 #pragma unroll 1
                 for (int c0 = (rlo * w) & ~63; c0 < a_hi; c0 += 64 * RPI) {
-                    T u[RPI][SZ][MAXP];
+                    T u[RPI][SZ][NPU];
                     int pp[RPI];
                     bool in[RPI];
 #pragma unroll
@@ -907,7 +910,7 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                         for (int z = 0; z < SZ; ++z) {
                             const bool okz = in[r] && (unsigned)(bz0 + z) < (unsigned)w;
 #pragma unroll
-                            for (int i = 0; i < MAXP; ++i)
+                            for (int i = 0; i < NPU; ++i)
                                 u[r][z][i] = bload<T>(rU, (okz && i < np) ? (unsigned)(i * w3 + (bz0 + z) * w2 + a) * TS : OOB);
                         }
                         pp[r] = in[r] ? (oy + by - uy0) * ux + (ox + bx - ux0) : 0;
@@ -924,7 +927,7 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
 #pragma unroll
                                 for (int m = 0; m < NG; ++m)
 #pragma unroll
-                                    for (int i = 0; i < MAXP; ++i)
+                                    for (int i = 0; i < NPU; ++i)
                                         acc[m * MAXP + i] += (double)v[m] * (double)u[r][z][i];
                             }
                         }

@@ -64,3 +64,84 @@ def test_extrapolation_coefficients():
     D0, s = rng.normal(size=50), rng.normal(size=50)
     dn = (1 + al) * D0 + (be - al) * (D0 - s) - be * (D0 - 2 * s)
     assert np.allclose(dn, D0 + s, rtol=0, atol=1e-12)
+
+
+class _System:
+    """Stand-in for sqdft_amd.System: 2 atoms of one type in a 20-bohr cell."""
+    n, n_atm = 4, 2
+
+    def __init__(self, x0):
+        self._x = np.array(x0, dtype=float)
+        self.info = {"L": 20.0, "bet": 1.0 / (md.KB * 3000.0)}
+
+    def atoms(self):
+        return self._x.copy(), np.zeros(2, dtype=np.int32)
+
+
+class _SCF:
+    """Stand-in for SCF with a harmonic potential energy and a density whose
+    difference from the accumulated atomic densities grows linearly with the
+    step (so a linear extrapolation of it is exact)."""
+    K = 0.05
+
+    def __init__(self, x0):
+        self.S = _System(x0)
+        self.x0 = np.array(x0, dtype=float)
+        self.rho = np.zeros(64)
+        self.step = 0
+        self.acc = np.zeros(64)
+        self.guesses = []
+
+    def set_positions(self, x):
+        self.x = np.array(x)
+        self.step += 1
+        self.rho_at = np.full(64, 0.5) + 0.01 * self.step
+        self.acc = self.acc + self.rho_at
+
+    def set_density_guess(self, g, keep_phi=True):
+        self.guesses.append((self.step, np.array(g), keep_phi))
+
+    def initial_potential(self):
+        pass
+
+    def run(self, **kw):
+        # converged density: accumulated rho_at + D0 + s * step
+        self.rho = self.acc + 1.0 + 0.1 * self.step
+        d = self.x - self.x0
+        return {"free_energy": 0.5 * self.K * np.sum(d * d) / 2, "iterations": 1, "converged": True}
+
+    def ForcesTotal(self):
+        return -self.K * (self.x - self.x0)
+
+
+def test_nve_density_guesses():
+    # free flight (no force): the atoms move on straight lines, so the fitted
+    # alpha + beta is 1 and a linearly growing density difference is
+    # extrapolated exactly
+    x0 = [[5.0, 5.0, 5.0], [9.0, 9.0, 9.0]]
+    scf = _SCF(x0)
+    scf.K = 0.0
+    dyn = md.NVE(scf, masses=[1.0], time_step=0.5, T_ion=3000.0)
+    dyn.run(8)
+    # rho_at at step 1, the last density at steps 2-3, then the extrapolation
+    acc = np.zeros(64)
+    for k, g, keep in scf.guesses:
+        acc = acc + (0.5 + 0.01 * k)
+        if k == 1:
+            assert np.allclose(g, acc) and not keep
+        elif k <= 3:
+            assert np.allclose(g, acc - (0.5 + 0.01 * k) + 1.0 + 0.1 * (k - 1)) and keep
+        else:
+            assert np.allclose(g, acc + 1.0 + 0.1 * k, rtol=0, atol=1e-6), k
+
+
+def test_nve_leapfrog_conserves_energy():
+    # leapfrog on a harmonic potential: the energy oscillates at O(dt^2)
+    # without drift, and the momentum stays zero (net force removed)
+    scf = _SCF([[5.0, 5.0, 5.0], [9.0, 9.0, 9.0]])
+    dyn = md.NVE(scf, masses=[1.0], time_step=0.5, T_ion=3000.0)
+    recs = dyn.run(40)
+    te = np.array([r["TE"] for r in recs])
+    wdt = np.sqrt(_SCF.K / md.AMU) * 0.5          # omega dt of the oscillator
+    assert np.max(np.abs(te - te[0])) < 0.5 * wdt ** 2 * np.max([r["KE"] for r in recs])
+    assert max(np.max(np.abs(r["velocity_sum"])) for r in recs) < 1e-12

@@ -45,7 +45,8 @@ extern "C" {
 #define SQG_E_BRENT_ITMAX   -8    /* Brent exceeded 100 iterations (sq.cpp:1400-1401) */
 #define SQG_E_DENOM         -9    /* a KB denominator is zero (nonlocal.cpp:163) */
 #define SQG_E_CUBE          -10   /* 2*nloc+1 above what one workgroup can march (43) */
-#define SQG_E_OVERFLOW      -11   /* per-node image list larger than the build limit */
+#define SQG_E_OVERFLOW      -11   /* a table above a build limit (image list, ELL
+                                     slots, a 2 GiB buffer range) */
 #define SQG_E_NOSTATE       -12   /* forces/getters called before sqg_density */
 #define SQG_E_FDN           -13   /* FD order other than 12 */
 #define SQG_E_SPLINE        -14   /* spline abscissa outside the radial grid (spline.cpp:17-22) */
@@ -70,7 +71,9 @@ typedef struct sqg_params {
     int fermi_alg;            /* 1 Brent, 2 Newton-Raphson */
     int precision;            /* SQG_FP64 or SQG_MIXED */
     int batch_nodes;          /* FD nodes per launch (0: all nodes, capped by memory) */
-    int ckpt_interval;        /* adjoint checkpoint interval m (reserved) */
+    int ckpt_interval;        /* adjoint checkpoint interval m: sqg_adjoint_veff keeps
+                                 the recurrence pair every m steps and recomputes the
+                                 vectors in between (<= 0 or > npl/2: npl/2) */
     int device;               /* HIP device ordinal */
 } sqg_params;
 

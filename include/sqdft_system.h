@@ -76,7 +76,8 @@ typedef struct sqdft_type_info {
     double rc, rz;           /* grid-rounded radii (readfiles.cpp:1121-1124) */
     double denom[4];         /* Kleinman-Bylander denominators per l (carry 1/h^3) */
     int size;                /* radial grid length (incl. r = 0) */
-    double rb;               /* pseudocharge radius (bohr) */
+    double rb;               /* pseudocharge radius as given (bohr); the replicas use
+                                ceil(rb/h)*h (src/readfiles.cpp:512) */
 } sqdft_type_info;
 
 typedef struct sqdft_system sqdft_system;
@@ -119,9 +120,10 @@ int  sqdft_system_udv(const sqdft_system *S, double *udv);
 int  sqdft_system_pseudocharge(const sqdft_system *S, double *b, double *rho, double *ncharge,
                                double *eself, double *ecorr, double *b_tilda, double *vc);
 
-/* Periodic replicas whose rb-box (half-width round(rb/h) nodes around the
- * nearest node) meets the domain [0, n-1]^3, in Replica_atoms' order (type,
- * atom, z-shift, y-shift, x-shift; src/initialize.cpp:21-200).  Call with
+/* Periodic replicas whose rb-box (half-width round(rb'/h) nodes around the
+ * nearest node, rb' = ceil(rb/h)*h as src/readfiles.cpp:512 rounds it)
+ * meets the domain [0, n-1]^3, in Replica_atoms' order (type, atom,
+ * z-shift, y-shift, x-shift; src/initialize.cpp:21-200).  Call with
  * NULL arrays to get the count in *n_img.  xyz[3*n_img], type[n_img],
  * atom[n_img] (global index in type order), box[6*n_img] = the intersection
  * lo x,y,z then hi x,y,z (inclusive node indices). */

@@ -352,12 +352,14 @@ def total_energy(h, phi, b, rho, vxc, ebs, eent, eself, ecorr, n_atm):
 def periodic_pulay(DX, DF, res, beta):
     """The Anderson update of AndersonExtrapolation (anderson.cpp:20-100) in
     the reference's loop order: FtF_ij = sum_k DF_i[k] DF_j[k], Ftf_j =
-    sum_k DF_j[k] res[k], x = pinv(FtF) Ftf with only exactly zero singular
-    values dropped, am[k] = sum_j (DX_j[k] + beta DF_j[k]) x_j."""
+    sum_k DF_j[k] res[k], x = pinv(FtF) Ftf with the singular values below
+    m * max(w) * 2.22044605e-16 zeroed (anderson.cpp:392-397) and zero ones
+    dropped (anderson.cpp:134-142), am[k] = sum_j (DX_j[k] + beta DF_j[k]) x_j."""
     m = DF.shape[0]
     FtF = np.array([[float(np.dot(DF[i], DF[j])) for j in range(m)] for i in range(m)])
     Ftf = np.array([float(np.dot(DF[j], res)) for j in range(m)])
     U, s, Vt = np.linalg.svd(FtF)
+    s = np.where(s < m * s.max() * 2.22044605e-16, 0.0, s)
     c = np.array([(U[:, j] @ Ftf) / s[j] if s[j] != 0 else 0.0 for j in range(m)])
     x = Vt.T @ c
     am = np.zeros_like(res)

@@ -1368,9 +1368,9 @@ int sqo_forces_nonlocal(const sqo_sys *S, const double *veff, const double *chi,
 /* ------------------------------------------------------------------ */
 
 /* x = pinv(A) b for the symmetric m x m matrix A (column major), by a
- * one-sided Jacobi SVD of A; singular values that come out exactly zero
- * are skipped, as anderson.cpp:134-142 skips w_j == 0 (a zero history
- * column gives an exactly zero row and column). */
+ * one-sided Jacobi SVD of A; singular values below m * max(w) *
+ * 2.22044605e-16 are zeroed (anderson.cpp:392-397) and zero ones skipped
+ * (anderson.cpp:134-142). */
 static void pinv_times_vec(const double *Ac, const double *b, double *x, int m)
 {
     double U[64], V[64], w[8];
@@ -1404,11 +1404,16 @@ static void pinv_times_vec(const double *Ac, const double *b, double *x, int m)
             }
         if (off < 1e-15) break;
     }
+    double wmax = 0.0;
     for (int j = 0; j < m; j++) {
         double s = 0;
         for (int i = 0; i < m; i++) s += U[j * m + i] * U[j * m + i];
         w[j] = sqrt(s);
+        if (w[j] > wmax) wmax = w[j];
     }
+    const double wmin = m * wmax * (2.22044605e-16);
+    for (int j = 0; j < m; j++)
+        if (w[j] < wmin) w[j] = 0.0;
     double tmp[8];
     for (int j = 0; j < m; j++) {   /* (U_j/w_j)' b / w_j */
         double s = 0.0;
@@ -1579,13 +1584,14 @@ static int rb_images(const sqo_sys *S, const double *rb, rb_img **out)
     rb_img *v = (rb_img *)malloc(cap * sizeof(rb_img));
     const double L = S->L, h = S->h;
     for (int t = 0; t < S->n_typ; t++) {
-        int Rb = (int)round(rb[t] / h);
+        const double rbt = ceil(rb[t] / h) * h;   /* rb rounded up to the mesh when read (readfiles.cpp:512) */
+        int Rb = (int)round(rbt / h);
         for (int q = 0; q < S->natm[t]; q++, a++) {
             double c[3] = {S->ax[a], S->ay[a], S->az[a]};
             int pm[3], mm[3];
             for (int d = 0; d < 3; d++) {
-                pm[d] = (int)floor(((L + rb[t] - c[d]) / L) + TEMP_TOL);
-                mm[d] = (int)ceil(((-rb[t] - c[d]) / L) - TEMP_TOL);
+                pm[d] = (int)floor(((L + rbt - c[d]) / L) + TEMP_TOL);
+                mm[d] = (int)ceil(((-rbt - c[d]) / L) - TEMP_TOL);
             }
             for (int rr = mm[2]; rr <= pm[2]; rr++)
                 for (int qq = mm[1]; qq <= pm[1]; qq++)

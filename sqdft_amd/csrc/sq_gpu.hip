@@ -513,15 +513,6 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
         (void)hipEventRecord(ph1, c->st);
         c->ev[sqg_ctx::EV_LANCZOS].push_back({ph0, ph1, 0.0});
     }
-    if (std::getenv("SQDFT_DEBUG_LANCZOS")) {
-        // diagnostic: alpha/beta of the first node of the batch
-        std::vector<double> ha(MAXIT + 2), hb(MAXIT + 2);
-        HIPCHK(hipMemcpy2D(ha.data(), sizeof(double), c->aa.p, (size_t)B * sizeof(double), sizeof(double),
-                           ha.size(), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy2D(hb.data(), sizeof(double), c->bb.p, (size_t)B * sizeof(double), sizeof(double),
-                           hb.size(), hipMemcpyDeviceToHost));
-        for (int i = 0; i < 120; ++i) std::fprintf(stderr, "lanczos b0 pass %d a=%.17g b=%.17g\n", i, ha[i], hb[i]);
-    }
     return SQG_OK;
 }
 
@@ -889,6 +880,8 @@ int sqg_create(const sqg_params *p, sqg_ctx **out)
     if (p->n < 1 || p->nloc < 0 || p->h <= 0) return SQG_E_ARG;
     if (2 * p->nloc + 1 > MAXM) return SQG_E_CUBE;
     if (p->precision != SQG_FP64 && p->precision != SQG_MIXED) return SQG_E_ARG;
+    // Veff is read through a buffer resource whose byte range is 32-bit
+    if ((size_t)p->n * p->n * p->n * sizeof(double) >= 0x80000000u) return SQG_E_OVERFLOW;
     std::unique_ptr<sqg_ctx> c(new sqg_ctx());
     c->p = *p;
     HIPCHK(hipSetDevice(p->device));
@@ -995,6 +988,9 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
     int kmax = 0;
     for (size_t g = 0; g < E3; ++g) kmax = std::max(kmax, cnt[g]);
     if (kmax > 255) return SQG_E_OVERFLOW;
+    // the kernels read the ELL through buffer resources with a 32-bit byte
+    // range (out-of-range loads return zero), so each table stays below 2 GiB
+    if ((size_t)std::max(kmax, 1) * E3 * sizeof(double) >= 0x80000000u) return SQG_E_OVERFLOW;
     const int KE = std::max(kmax, 1);
     std::vector<unsigned char> ecnt(E3, 0);
     std::vector<int> eip((size_t)KE * E3, 0);
@@ -1043,7 +1039,6 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
     if (st) return SQG_E_HIP;
     // the nonlocal apply keeps every projection of its node in LDS (160 KB per CU)
     if ((size_t)std::max(c->n_ip, 1) * sizeof(double) > 160 * 1024) return SQG_E_OVERFLOW;
-    if ((size_t)std::max(kmax, 1) * E3 * sizeof(double) >= 0x80000000u) return SQG_E_OVERFLOW;
     c->geom = true;
     return SQG_OK;
 }

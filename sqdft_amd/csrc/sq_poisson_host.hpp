@@ -2,9 +2,9 @@
 // potential (included by sq_gpu.hip inside its anonymous namespace).
 
 // x = pinv(A) b for a symmetric m x m matrix (column major) through a
-// one-sided Jacobi SVD; exactly zero singular values (the zero rows and
-// columns of history slots not filled yet) are skipped, as
-// PseudoInverseTimesVec does (src/anderson.cpp:105-163).
+// one-sided Jacobi SVD.  Singular values below m * max(w) * 2.22044605e-16
+// are set to zero and zero ones skipped, as SingularValueDecomp and
+// PseudoInverseTimesVec do (src/anderson.cpp:392-397, :134-142).
 void pinv_solve(const double *Ac, const double *b, double *x, int m)
 {
     double W[64], V[64];
@@ -37,15 +37,19 @@ void pinv_solve(const double *Ac, const double *b, double *x, int m)
             }
         if (worst < 1e-15) break;
     }
-    double coef[8];
+    double coef[8], sv[8], wmax = 0.0;
     for (int j = 0; j < m; ++j) {
         double s2 = 0;
         for (int i = 0; i < m; ++i) s2 += W[j * m + i] * W[j * m + i];
-        const double sv = std::sqrt(s2);
+        sv[j] = std::sqrt(s2);
+        if (sv[j] > wmax) wmax = sv[j];
+    }
+    const double wmin = m * wmax * (2.22044605e-16);
+    for (int j = 0; j < m; ++j) {
         double s = 0.0;
-        if (sv != 0.0) {
-            for (int i = 0; i < m; ++i) s += W[j * m + i] / sv * b[i];
-            s /= sv;
+        if (sv[j] >= wmin && sv[j] != 0.0) {
+            for (int i = 0; i < m; ++i) s += W[j * m + i] / sv[j] * b[i];
+            s /= sv[j];
         }
         coef[j] = s;
     }

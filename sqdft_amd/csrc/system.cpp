@@ -319,8 +319,11 @@ int build(const sqdft_spec *spec, sqdft_system *S)
             }
         for (int l = 0; l < 4; ++l) ti.denom[l] = P.denom[l];
         if (ti.nproj > I.max_nproj) I.max_nproj = ti.nproj;
-        ti.rb = T.rb;
-        S->rb.push_back(T.rb);
+        // rb is rounded up to a whole number of mesh intervals when the atoms
+        // are read (src/readfiles.cpp:512); the replica boxes then use
+        // round(rb/h) nodes (src/initialize.cpp:68-70)
+        ti.rb = T.rb;   // as given; the replica search uses the rounded value
+        S->rb.push_back(T.rb > 0 ? std::ceil(T.rb / I.h) * I.h : T.rb);
         for (int a = 0; a < T.natm; ++a) {
             S->atom_xyz.push_back(T.xyz[3 * a]);
             S->atom_xyz.push_back(T.xyz[3 * a + 1]);

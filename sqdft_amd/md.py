@@ -22,7 +22,7 @@ import time
 
 import numpy as np
 
-from .scf import SCF
+from .scf import SCF, pinv_times_vec
 
 AMU = 1.066572305                     # Ha fs^2 / bohr^2 per amu
 KB = 8.617343e-5 / 27.211384523       # Ha / K (src/readfiles.cpp:309-311)
@@ -82,22 +82,13 @@ def wrap_around(xyz: np.ndarray, L: float) -> np.ndarray:
     return x
 
 
-def pinv2_times_vec(A: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """PseudoInverseTimesVec for the 2x2 system of the extrapolation
-    (exactly-zero singular values dropped)."""
-    U, s, Vt = np.linalg.svd(A)
-    c = U.T @ b
-    c = np.where(s != 0, c / np.where(s != 0, s, 1.0), 0.0)
-    return Vt.T @ c
-
-
 def extrapolation_coefficients(r0: np.ndarray, r1: np.ndarray, r2: np.ndarray, rnew: np.ndarray) -> tuple:
     """alpha, beta of ChargeExtrapolation_MD (src/md.cpp:186-276): least
     squares of rnew - r0 on (r0 - r1, r1 - r2) over all coordinates."""
     d01, d12, dn = (r0 - r1).ravel(), (r1 - r2).ravel(), (rnew - r0).ravel()
     FtF = np.array([[d01 @ d01, d01 @ d12], [d01 @ d12, d12 @ d12]])
     Ftf = np.array([d01 @ dn, d12 @ dn])
-    a, b = pinv2_times_vec(FtF, Ftf)
+    a, b = pinv_times_vec(FtF, Ftf)
     return float(a), float(b)
 
 

@@ -50,16 +50,22 @@ def gmres(matvec, rhs: np.ndarray, k: int) -> tuple[np.ndarray, float]:
     return sum(y[i] * Q[i] for i in range(kk)), res
 
 
+def pinv_times_vec(A: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """x = pinv(A) b for a small symmetric matrix as PseudoInverseTimesVec
+    (src/anderson.cpp:100-163) computes it: singular values below
+    m * max(w) * 2.22044605e-16 are set to zero (SingularValueDecomp,
+    src/anderson.cpp:392-397) and zero ones dropped."""
+    U, s, Vt = np.linalg.svd(A)
+    s = np.where(s < len(s) * s.max() * 2.22044605e-16, 0.0, s)
+    c = U.T @ b
+    c = np.where(s != 0, c / np.where(s != 0, s, 1.0), 0.0)
+    return Vt.T @ c
+
+
 def anderson_extrapolation(DX: np.ndarray, DF: np.ndarray, res: np.ndarray, beta: float) -> np.ndarray:
     """Anderson update of AndersonExtrapolation (src/anderson.cpp:20-100):
-    x = pinv(DF DF') DF res (exactly zero singular values dropped, as
-    PseudoInverseTimesVec does), update = sum_j (DX_j + beta DF_j) x_j."""
-    FtF = DF @ DF.T
-    Ftf = DF @ res
-    U, s, Vt = np.linalg.svd(FtF)
-    c = U.T @ Ftf
-    c = np.where(s != 0, c / np.where(s != 0, s, 1.0), 0.0)
-    x = Vt.T @ c
+    x = pinv(DF DF') DF res, update = sum_j (DX_j + beta DF_j) x_j."""
+    x = pinv_times_vec(DF @ DF.T, DF @ res)
     return (DX + beta * DF).T @ x
 
 
@@ -299,6 +305,7 @@ class SCF:
                 self.veff = vnew
             history.append({"lambda_f": last["lambda_f"], "etot": last["etot"], "scf_err": err,
                             "poisson": last["poisson"], "fermi_iters": last["stats"]["fermi_iters"],
+                            "lanczos0": last["stats"]["lanczos_iters_node0"], "brent": last["stats"]["fermi_iters"],
                             "t_sq": last["t_sq"], "t_rest": last["t_rest"], "gmres_res": gres})
             count += 1
         return {"iterations": count - 1, "converged": err <= pp["scf_tol"], "free_energy": last["etot"],

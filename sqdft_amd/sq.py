@@ -29,7 +29,8 @@ PRECISIONS = {"fp64": 0, "mixed": 1}
 
 class SpectralQuadrature:
     def __init__(self, system: System, precision: str = "fp64", batch_nodes: int = 0,
-                 device: int = 0, ckpt_interval: int = 8, fermi_tol: float | None = None):
+                 device: int = 0, ckpt_interval: int = 8, fermi_tol: float | None = None,
+                 fermi_alg: int | None = None, lanczos_tol: float | None = None):
         self._lib = L.load()
         self.system = system
         info = system.info
@@ -49,9 +50,9 @@ class SpectralQuadrature:
         p.nloc = info["nloc"]
         p.npl = info["npl"]
         p.bet = info["bet"]
-        p.lanczos_tol = info["lanczos_tol"]
+        p.lanczos_tol = info["lanczos_tol"] if lanczos_tol is None else lanczos_tol
         p.fermi_tol = info["fermi_tol"] if fermi_tol is None else fermi_tol
-        p.fermi_alg = info["fermi_alg"]
+        p.fermi_alg = info["fermi_alg"] if fermi_alg is None else int(fermi_alg)
         p.precision = PRECISIONS[precision]
         p.batch_nodes = batch_nodes
         p.ckpt_interval = ckpt_interval

@@ -94,6 +94,79 @@ def test_density_matches_oracle(cases, name):
     assert H.rel(out2["rho"], ref2["rho"]) < FP64_TOL
 
 
+@pytest.mark.parametrize("name", ["small_al", "test_lih"])
+def test_newton_fermi_matches_oracle(cases, name):
+    """fermi_alg = 2: NewtonRaphson on NeConstraint_derv (src/sq.cpp:1407-1490),
+    started at 0.2 in the first SCF iteration and at the previous Fermi level
+    afterwards (src/sq.cpp:722-725)."""
+    from oracle import OracleSystem
+    S, osys1, veff = cases[name]
+    say("newton", name)
+    Rcut, T = SYSTEMS[name]()[1:]
+    info = S.info
+    osys = OracleSystem(info["n"], info["L"], H.types_of(S), Rcut, info["npl"], T, h=info["h"],
+                        lanczos_tol=info["lanczos_tol"], fermi_tol=info["fermi_tol"], fermi_alg=2)
+    ref = osys.density(veff, scf_iter=1, nthreads=0)
+    sq = SpectralQuadrature(S, precision="fp64", fermi_alg=2)
+    out = sq.density(veff, scf_iter=1)
+    say("newton", name, "its", out["stats"]["fermi_iters"], ref["fermi_iters"], "lambda_f", out["lambda_f"],
+        ref["lambda_f"])
+    assert out["stats"]["fermi_iters"] == ref["fermi_iters"]
+    assert out["lambda_f"] == pytest.approx(ref["lambda_f"], abs=FP64_TOL)
+    assert H.rel(out["rho"], ref["rho"]) < FP64_TOL
+    # Newton and Brent find the same root to the Fermi tolerance
+    b = osys1.density(veff, scf_iter=1, nthreads=0)
+    assert abs(out["lambda_f"] - b["lambda_f"]) < 10 * info["fermi_tol"]
+    ref2 = osys.density(veff + 0.01, scf_iter=2, lambda_f=ref["lambda_f"], nthreads=0)
+    out2 = sq.density(veff + 0.01, scf_iter=2, lambda_f=out["lambda_f"])
+    assert out2["stats"]["fermi_iters"] == ref2["fermi_iters"]
+    assert out2["lambda_f"] == pytest.approx(ref2["lambda_f"], abs=FP64_TOL)
+    assert H.rel(out2["rho"], ref2["rho"]) < FP64_TOL
+    sq.close()
+
+
+def test_exit_paths_return_status(cases):
+    """The reference's exit paths as status codes, each forced from the host
+    with valid memory and the same condition the oracle reports in
+    test_status_codes.py: Brent bracket (sq.cpp:1307-1311), Brent ITMAX
+    (:1400-1401), negative density (:764-765) and the Lanczos max_iter
+    warning (:1073-1074)."""
+    from sqdft_amd._lib import SQError
+    from test_status_codes import neg_rho_case
+    S, _, veff = cases["small_al"]
+    say("status: bracket")
+    sq = SpectralQuadrature(S, precision="fp64")
+    good = sq.density(veff, scf_iter=1)
+    with pytest.raises(SQError) as e:
+        sq.density(veff, scf_iter=2, lambda_f=good["lambda_f"] + 10.0)
+    assert e.value.code == -7
+    # the context stays usable after an error return
+    again = sq.density(veff, scf_iter=2, lambda_f=good["lambda_f"])
+    assert again["lambda_f"] == pytest.approx(good["lambda_f"], abs=1e-6)
+    sq.close()
+    say("status: brent itmax")
+    sq = SpectralQuadrature(S, precision="fp64", fermi_tol=-1.0)
+    with pytest.raises(SQError) as e:
+        sq.density(veff, scf_iter=1)
+    assert e.value.code == -8
+    sq.close()
+    say("status: negative density")
+    Sn, _, _, vn = neg_rho_case()
+    sq = SpectralQuadrature(Sn, precision="fp64")
+    with pytest.raises(SQError) as e:
+        sq.density(vn, scf_iter=1)
+    assert e.value.code == -5
+    sq.close()
+    say("status: lanczos max_iter")
+    Sl, _, _ = H.small_al(npl=8)
+    vl = smooth_veff(Sl)
+    sq = SpectralQuadrature(Sl, precision="fp64", lanczos_tol=-1.0)
+    out = sq.density(vl, scf_iter=1)
+    assert out["status"] == 1
+    assert np.all(sq.node_data()["iters"] == 500)
+    sq.close()
+
+
 @pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])
 def test_nonlocal_forces_match_oracle(cases, name):
     S, osys, veff = cases[name]

@@ -111,6 +111,26 @@ def test_total_forces_on_gpu_match_log(al):
     scf.close()
 
 
+def test_lih_scf_and_forces_on_gpu_match_log():
+    """The reference's LiH test (two types, Li p projectors) through the
+    C-ABI: the first Poisson solve, all 7 SCF iterations with Periodic Pulay
+    mixing and the printed forces (results_LiH:83-149)."""
+    from test_scf_oracle import LIH_FIRST_POISSON, LIH_FORCES, check_against_lih_log
+    S, _, _ = H.test_lih()
+    say("scf: LiH log on the GPU path")
+    scf = SCF(S)
+    scf.initial_potential()
+    assert (scf.poisson_log[0][0], "%g" % scf.poisson_log[0][1]) == LIH_FIRST_POISSON
+    out = scf.run()
+    say("scf: LiH", out["iterations"], "iterations, free energy", out["free_energy"])
+    assert out["converged"] and out["iterations"] == 7
+    check_against_lih_log(out["history"])
+    f = scf.ForcesTotal()
+    say("max |f - log|", float(np.max(np.abs(f - LIH_FORCES))))
+    assert np.max(np.abs(f - LIH_FORCES)) < 1e-10
+    scf.close()
+
+
 def test_local_forces_two_types_match_oracle():
     """LiH (two types, different radial grids and rb): the GPU local forces on
     the starting phi against the oracle."""

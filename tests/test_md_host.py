@@ -66,6 +66,45 @@ def test_extrapolation_coefficients():
     assert np.allclose(dn, D0 + s, rtol=0, atol=1e-12)
 
 
+def test_extrapolation_near_straight_line():
+    """Free flight with positions accumulated step by step as NVE.step does:
+    the two history columns agree up to rounding, the Gram matrix is singular
+    to working precision and its small singular value is zeroed
+    (src/anderson.cpp:392-397), so the coefficients are the minimum-norm
+    alpha = beta = 1/2 of the exact straight line."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        x = rng.uniform(0, 20, size=(8, 3))
+        v = rng.normal(scale=1e-3, size=(8, 3))
+        hist = [x.copy()]
+        for _k in range(3):
+            x = x + v * 41.341
+            hist.append(x.copy())
+        al, be = md.extrapolation_coefficients(hist[2], hist[1], hist[0], hist[3])
+        assert abs(al - 0.5) < 1e-6 and abs(be - 0.5) < 1e-6, (al, be)
+
+
+def test_pinv_zeroes_small_singular_values():
+    """A Pulay history with two columns equal up to rounding gives the same
+    update as the exactly rank-deficient history (the reference's threshold
+    m * max(w) * eps), not one blown up by 1/w_min."""
+    from sqdft_amd.scf import anderson_extrapolation, pinv_times_vec
+    rng = np.random.default_rng(6)
+    N = 400
+    f, g, res = rng.normal(size=N), rng.normal(size=N), rng.normal(size=N)
+    DF_exact = np.stack([f, g, f])
+    DF_near = np.stack([f, g, f * (1 + 1e-16 * rng.choice([-1.0, 1.0], size=N))])
+    DX = rng.normal(size=(3, N))
+    a = anderson_extrapolation(DX, DF_exact, res, 0.1)
+    b = anderson_extrapolation(DX, DF_near, res, 0.1)
+    assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(a))
+    # a full-rank system is solved exactly
+    A = rng.normal(size=(4, 4))
+    A = A @ A.T + np.eye(4)
+    y = rng.normal(size=4)
+    assert np.allclose(A @ pinv_times_vec(A, y), y, rtol=0, atol=1e-12)
+
+
 class _System:
     """Stand-in for sqdft_amd.System: 2 atoms of one type in a 20-bohr cell."""
     n, n_atm = 4, 2

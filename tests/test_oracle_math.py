@@ -151,3 +151,63 @@ def test_kb_denominator_against_scipy_spline(psd_dir):
     rr = np.arange(1, int(np.ceil(ti["rc"] / dr)) + 1) * dr
     den = np.sum(su(rr) ** 2 * (sv(rr) - sl(rr)) / rr) * dr / h ** 3
     assert ti["denom"][l] == pytest.approx(den, rel=1e-8)
+
+
+def _lih():
+    import helpers as H
+    S, _, _ = H.test_lih()
+    return S
+
+
+def test_kb_denominator_p_channel_li():
+    # the same rectangle rule for Li's l = 1 channel (lloc = 0): the
+    # projector channel of the two-type cell
+    S = _lih()
+    ti, R, h = S.type_info(0), S.radial(0), S.info["h"]
+    assert (ti["lloc"], ti["lmax"], ti["nproj"]) == (0, 1, 3)
+    r = R["r"]
+    su = CubicSpline(r, R["rU"][1], bc_type="not-a-knot")
+    sv = CubicSpline(r, R["rV"][1], bc_type="not-a-knot")
+    sl = CubicSpline(r, R["rVloc"], bc_type="not-a-knot")
+    dr = 1e-4
+    rr = np.arange(1, int(np.ceil(ti["rc"] / dr)) + 1) * dr
+    den = np.sum(su(rr) ** 2 * (sv(rr) - sl(rr)) / rr) * dr / h ** 3
+    assert ti["denom"][1] == pytest.approx(den, rel=1e-8)
+    assert ti["denom"][0] == 0.0   # the local channel has no projector
+
+
+def test_p_projector_tables_against_analytic_form():
+    """UdV_{J,1m}(p) = U_1(r) Y_1m(p - R_J) (V_1(r) - V_loc(r)) at points of
+    Li image tables (src/nonlocal.cpp:317-518), with an independent
+    not-a-knot spline of rU, rV and the real l = 1 harmonics written from
+    sqrt(3/(4 pi)) (y, z, x)/r for m = -1, 0, 1 (nonlocal.cpp:255-262)."""
+    S = _lih()
+    ti, R, h = S.type_info(0), S.radial(0), S.info["h"]
+    nc = ti["nlocc"]
+    w = 2 * nc + 1
+    r = R["r"]
+    su = CubicSpline(r, R["rU"][1], bc_type="not-a-knot")
+    sv = CubicSpline(r, R["rV"][1], bc_type="not-a-knot")
+    sl = CubicSpline(r, R["rVloc"], bc_type="not-a-knot")
+    im, udv = S.images(), S.udv()
+    li = np.flatnonzero(im["type"] == 0)
+    rng = np.random.Generator(np.random.PCG64(3))
+    c1 = np.sqrt(3.0 / (4.0 * np.pi))
+    checked = 0
+    for J in rng.choice(li, size=6, replace=False):
+        tab = udv[im["udv_off"][J]: im["udv_off"][J] + 3 * w ** 3].reshape(3, w, w, w)
+        scale = np.max(np.abs(tab))
+        assert scale > 0
+        for _ in range(40):
+            k, j, i = rng.integers(0, w, size=3)
+            p = (im["nn"][J] + np.array([i, j, k]) - nc) * h
+            d = p - im["xyz"][J]
+            rad = np.sqrt(d @ d)
+            if rad < 1e-6:
+                continue
+            radial = su(rad) / rad * (sv(rad) - sl(rad)) / rad
+            want = c1 * radial * np.array([d[1], d[2], d[0]]) / rad
+            got = tab[:, k, j, i]
+            assert np.max(np.abs(got - want)) < 1e-6 * scale, (J, i, j, k, got, want)
+            checked += 1
+    assert checked > 200

@@ -140,11 +140,50 @@ AL_FORCES = np.array([
 ])
 
 
-@pytest.fixture(scope="module")
-def oracle_scf(al):
-    """The oracle's SCF on the Al test (SCF_iteration, src/scf.cpp:84-330):
-    SQ density, energy, AAJ Poisson, LDA potential, Periodic Pulay mixing."""
-    S, Rcut, T = al
+# results_LiH:84-131 (the two-type cell with Li p projectors, logged on 125
+# ranks): SCF_err, Fermi level, energy, the Poisson iterations / residual
+# after each SQ call, the node-0 Lanczos count and the Brent count
+LIH_FIRST_POISSON = (55, "7.53331e-07")
+LIH_LOG = [
+    ("0.018343", -1.008217027252, -0.532706606432, 49, "6.71094e-07", 49, 18),
+    ("0.0158935", -1.006167228746, -0.564408401597, 37, "3.36836e-07", 49, 10),
+    ("0.00198387", -0.993069954064, -0.551687196208, 43, "3.974e-07", 49, 9),
+    ("0.0016848", -0.992986215798, -0.545518604225, 25, "8.16063e-07", 49, 10),
+    ("0.00042707", -0.992378129314, -0.544915840292, 31, "9.47354e-07", 49, 10),
+    ("0.000380234", -0.992412689712, -0.544386960546, 19, "5.83205e-07", 49, 10),
+    ("4.60187e-05", -0.992736808027, -0.544697801270, 25, "6.20585e-07", 49, 10),
+]
+# results_LiH:142-149: atomic forces after the SCF, fx fy fz per atom (Li then H)
+LIH_FORCES = np.array([
+    [-0.017820845518, -0.011544988780, -0.009466682119],
+    [0.004475176583, -0.001107721775, 0.003459663659],
+    [0.004741379755, 0.004057032876, -0.000993951921],
+    [-0.001313577284, 0.003255894016, 0.002943393242],
+    [0.013219935427, -0.001938348445, -0.001670685041],
+    [-0.001875628464, 0.008358768547, -0.001222381764],
+    [-0.001677027784, -0.001270492866, 0.006794020517],
+    [0.000180104132, 0.000141726862, 0.000124005296],
+])
+
+
+def check_against_lih_log(history):
+    # the logged run summed over 125 ranks, in another order than one domain;
+    # seven SCF iterations carry those ~1e-16 differences to a few units of
+    # the last printed digit (1e-12) by the end (measured: 4e-12 in the
+    # Fermi level, 3.1e-12 in the energy of iteration 7)
+    assert len(history) == len(LIH_LOG)
+    for h, (err, lf, E, pits, pres, lz, nb) in zip(history, LIH_LOG):
+        assert "%g" % h["scf_err"] == err
+        assert abs(h["lambda_f"] - lf) < 6e-12, (h["lambda_f"], lf)
+        assert abs(h["etot"] - E) < 5e-12, (h["etot"], E)
+        assert (h["poisson"][0], "%g" % h["poisson"][1]) == (pits, pres)
+        if "lanczos0" in h:
+            assert (h["lanczos0"], h["brent"]) == (lz, nb)
+
+
+def run_oracle_scf(S, Rcut, T):
+    """The oracle's SCF (SCF_iteration, src/scf.cpp:84-330): SQ density,
+    energy, AAJ Poisson, LDA potential, Periodic Pulay mixing."""
     osys = H.oracle_for(S, Rcut, T)
     info = S.info
     n, F, h = info["n"], info["FDn"], info["h"]
@@ -152,7 +191,7 @@ def oracle_scf(al):
     pc, pp = S.pseudocharge(), S.poisson_params()
     args = (pp["tol"], pp["maxiter"], pp["beta"], pp["m"], pp["p"])
     b = pc["b"]
-    phi, _, _ = O.poisson_aaj(n, F, lap, b + pc["rho"], np.ones(n ** 3), *args)
+    phi, pit0, prr0 = O.poisson_aaj(n, F, lap, b + pc["rho"], np.ones(n ** 3), *args)
     vxc = O.vxc_lda(pc["rho"])
     veff = phi + vxc
     m, p, beta = pp["m_scf"], pp["p_scf"], pp["beta_scf"]
@@ -167,7 +206,8 @@ def oracle_scf(al):
         vxc = O.vxc_lda(out["rho"])
         res = phi + vxc - veff_old
         err = np.sqrt(np.sum(res ** 2)) / np.sqrt(np.sum((phi + vxc) ** 2))
-        hist.append({"scf_err": err, "lambda_f": lf, "etot": E, "poisson": (pit, prr)})
+        hist.append({"scf_err": err, "lambda_f": lf, "etot": E, "poisson": (pit, prr),
+                     "lanczos0": int(out["iters"][0]), "brent": out["fermi_iters"]})
         vnew = veff_old + beta * res
         if count > 1:
             k = (count - 2) % m
@@ -179,7 +219,13 @@ def oracle_scf(al):
         if err > pp["scf_tol"]:
             veff = vnew
         count += 1
-    return {"osys": osys, "history": hist, "veff": veff, "phi": phi, "last": out}
+    return {"osys": osys, "history": hist, "veff": veff, "phi": phi, "last": out,
+            "first_poisson": (pit0, prr0), "ecorr": pc["ecorr"] / S.n_atm}
+
+
+@pytest.fixture(scope="module")
+def oracle_scf(al):
+    return run_oracle_scf(*al)
 
 
 def test_oracle_scf_converges_like_log(oracle_scf):
@@ -216,3 +262,22 @@ def test_oracle_total_forces_match_log(al, oracle_scf):
     f = fl + fnl + fc
     # printed with 12 decimals after an 8-rank sum
     assert np.max(np.abs(f - AL_FORCES)) < 2e-12, np.max(np.abs(f - AL_FORCES))
+
+
+def test_oracle_lih_scf_and_forces_match_log():
+    """The reference's LiH test end to end on the oracle: two atom types,
+    Li with three p projectors (lloc = 0), H local only, rb = 8 bohr rounded
+    up to 17 mesh intervals.  Energy correction, the first Poisson solve, all
+    7 SCF iterations and the printed forces (results_LiH:83-149)."""
+    S, Rcut, T = H.test_lih()
+    r = run_oracle_scf(S, Rcut, T)
+    assert abs(r["ecorr"] - (-0.00070193389666)) < 1e-12
+    assert (r["first_poisson"][0], "%g" % r["first_poisson"][1]) == LIH_FIRST_POISSON
+    check_against_lih_log(r["history"])
+    osys = r["osys"]
+    rb = np.array([t.rb for t in H.types_of(S)])
+    cd = osys.charge_density(rb)
+    fl, fc = osys.forces_local(rb, r["phi"], cd["b"], cd["b_tilda"], cd["vc"])
+    last = r["last"]
+    f = fl + fc + osys.forces_nonlocal(r["veff"], last["chi"], last["zee"], last["Ci"])
+    assert np.max(np.abs(f - LIH_FORCES)) < 2e-12, np.max(np.abs(f - LIH_FORCES))

@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -39,14 +40,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """The CPU share this process may use: its affinity set, capped by
+    OMP_NUM_THREADS when that is set (16 per GPU on the GPU hosts)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
 def cpu_baseline(system, veff, Rcut, T, seconds, seed=5):
-    """Times the FP64 oracle (OpenMP over nodes) on a seeded node sample."""
+    """Times the FP64 oracle's per-node loop (an -O3 build of the same source,
+    OpenMP over nodes on every thread of this process's CPU share) on a seeded
+    node sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402
     import helpers  # noqa: E402
 
+    libpath = oracle.use_timing_build()
     osys = helpers.oracle_for(system, Rcut, T)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     rng = np.random.Generator(np.random.PCG64(seed))
     # calibrate on one node per thread, then size the sample to ~`seconds`
     probe = rng.choice(system.nn, size=threads, replace=False).astype(np.int32)
@@ -60,9 +86,11 @@ def cpu_baseline(system, veff, Rcut, T, seconds, seed=5):
     osys.moments_nodes(veff, sample, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": sample.size / dt, "unit": "nodes/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "build": os.path.basename(libpath) + " (gcc -O3 -funroll-loops, "
+            + ("-march=x86-64-v4" if libpath.endswith("_v4.so") else "-march=x86-64-v3") + ", OpenMP)",
             "sample": f"{sample.size} seeded random nodes of the same workload (Lanczos + Chebyshev moments "
-                      f"per node, FP64 oracle sq_oracle.c, OpenMP over nodes), {dt:.1f} s; the Fermi solve "
-                      f"and rho assembly (<1% of the step) are excluded",
+                      f"per node, FP64 oracle sq_oracle.c, OpenMP over nodes on {threads} threads), {dt:.1f} s; "
+                      f"the Fermi solve and rho assembly (<1% of the step) are excluded",
             "seconds": dt}
 
 

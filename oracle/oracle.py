@@ -43,6 +43,29 @@ class Sys(C.Structure):
 _lib = None
 
 
+def _cpu_flags() -> set:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def use_timing_build() -> str:
+    """Switches this process to an -O3 timing build of the same source (for
+    the CPU baseline only; the checker stays the -ffp-contract=off build).
+    Must be called before the first load(); returns the library path."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("oracle already loaded; call use_timing_build() first")
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= _cpu_flags()
+    LIB = os.path.join(HERE, "libsq_oracle_v4.so" if v4 else "libsq_oracle_v3.so")
+    return LIB
+
+
 def load() -> C.CDLL:
     global _lib
     if _lib is not None:

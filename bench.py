@@ -1,13 +1,17 @@
-"""Benchmark of the SQ density step (the spectral-quadrature hot path).
+"""Benchmark of the SQ hot path (Clenshaw-Curtis spectral quadrature).
 
-A step is one ClenshawCurtisSpectralQuadrature call (sq.cpp:377-834) over
-every FD node of the workload: Lanczos bounds, Chebyshev moments, Fermi
-level, rho/Ebs/Eent.  Inputs (Veff, projector tables) are resident on the
-device before the timed region; the call returns after its stream has
-synchronised.  Prints one JSON line.
-
-Default workload: rocksalt LiH 3x3x3 (216 atoms), 48^3 FD nodes, 12th-order
-FD, T = 40,000 K, Rcut = 7 bohr (nloc = 15, 31^3 cube), npl = 64, FP64.
+A step is one pass of the per-node SQ loop (sq.cpp:530-661: Lanczos spectral
+bounds and Chebyshev moments of every node) over one batch of FD nodes, the
+unit the large workload is launched in: 8,192 nodes of the 864-atom FCC Al
+cell (96^3 FD nodes, 12th-order FD, nloc = 17, 35^3 cube, npl = 48,
+T = 58,000 K).  Batches are taken in the library's visiting order and cycle
+through the cell.  Veff and the projector tables are resident on the device;
+each step uploads Veff (7 MB) and returns the batch's moments, and its
+kernels dominate (99 %).  Then, once each: the whole density step of the cell
+(every batch, the Fermi level, rho) in FP64 and in mixed precision with the
+density error between them; an SCF iteration and the forces of the 216-atom
+LiH cell; SCF mixing and NVE steps on the reference's Al test input; the
+FP64 oracle on the host's cores.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -32,6 +36,8 @@ WORKLOADS = {
     "al32": ("FCC Al 2x2x2 (32 atoms), 32^3 nodes, FD order 12, nloc=17 (35^3 cube), npl=48, "
              "T=58000 K, seeded 0.1 bohr displacements"),
     "al256": ("FCC Al 4x4x4 (256 atoms), 64^3 nodes, FD order 12, nloc=17 (35^3 cube), npl=48, "
+              "T=58000 K, seeded 0.1 bohr displacements"),
+    "al864": ("FCC Al 6x6x6 (864 atoms), 96^3 nodes, FD order 12, nloc=17 (35^3 cube), npl=48, "
               "T=58000 K, seeded 0.1 bohr displacements"),
 }
 
@@ -129,16 +135,67 @@ def calibration_copy(nb: int) -> None:
     log(f"bench: calibration copy of {nb} bytes read + {nb} bytes written")
 
 
+KERNELS = [("k_zmarch<CHEB>", "Chebyshev recurrence: read t_k, t_{k-1}, write t_{k+1}", 3),
+           ("k_zmarch<LANCZOS>", "Lanczos pass: read r', v_{k-1}, write v_k, r'_{k+1}", 4),
+           ("projections<LANCZOS>", "projections of the Lanczos vector: read r', v_{k-1}", 2),
+           ("projections<CHEB>", "projections of t_k: read t_k", 1)]
+
+
+def roofline(sq, workload, precision, N):
+    """The hot kernels of the last call, timed with HIP events around every
+    launch on the library's stream; the dominant one is the roofline."""
+    rs = 8 if precision == "fp64" else 4
+    traffic = profiled_traffic(workload, precision)
+    rl = []
+    for w, (key, what, nvec) in enumerate(KERNELS):
+        k = sq.kernel_timing(w)
+        avg_ms = k["total_ms"] / max(k["launches"], 1)
+        alg = k["alg_bytes"] / max(k["launches"], 1)
+        rl.append({"kernel": f"{key} ({what})", "launches": k["launches"], "avg_ms": avg_ms,
+                   "total_ms": k["total_ms"], "alg_bytes_per_launch": alg,
+                   "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0,
+                   "alg_bytes_per_unit": f"{nvec} * {N} * {rs} B per node-step",
+                   "traffic": traffic.get(key)})
+    dom = int(np.argmax([r["total_ms"] for r in rl]))
+    d = rl[dom]
+    return {"bound": "hbm", "kernel": d["kernel"], "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["achieved_GBs"] / HBM_PEAK_GBS, "traffic": d["traffic"],
+            "alg_bytes_per_unit": d["alg_bytes_per_unit"], "launches": d["launches"], "avg_ms": d["avg_ms"],
+            "traffic_source": (f"rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a plain copy "
+                               f"(profiles/traffic_{workload}.md)") if d["traffic"] else None,
+            "others": [r for i, r in enumerate(rl) if i != dom]}
+
+
+def batch_steps(sq, veff, order, B, warmup, steps, label):
+    """Times `steps` batch steps after `warmup` untimed ones; returns
+    (seconds per step, nodes per step, last result)."""
+    nbat = (order.size + B - 1) // B
+    nodes_of = lambda i: order[(i % nbat) * B:(i % nbat) * B + B]
+    for i in range(warmup):
+        sq.node_moments(veff, nodes_of(i))
+        log(f"bench: {label} warmup {i}")
+    times, count, last = [], 0, None
+    for i in range(warmup, warmup + steps):
+        nd = nodes_of(i)
+        t0 = time.perf_counter()
+        last = sq.node_moments(veff, nd)
+        times.append(time.perf_counter() - t0)
+        count += nd.size
+        log(f"bench: {label} step {i - warmup} {times[-1]:.3f} s (Lanczos {last['iters'].min()}..{last['iters'].max()})")
+    return float(np.sum(times)) / steps, count / steps, last
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="lih216", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="al864", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--batch-nodes", type=int, default=0)
+    ap.add_argument("--batch-nodes", type=int, default=8192)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full", action="store_true", help="skip the whole-cell density steps")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -152,108 +209,80 @@ def main():
         sys.exit(2)
 
     from sqdft_amd import configs
-    from sqdft_amd.sq import SpectralQuadrature
+    from sqdft_amd.sq import SpectralQuadrature, node_order
 
     cfg = configs.CONFIGS[args.workload]
     log(f"bench: building {args.workload}")
     S = configs.build(args.workload, seed=0)
     veff = configs.smooth_veff(S)
-    sq = SpectralQuadrature(S, precision=args.precision, batch_nodes=args.batch_nodes)
     info = S.info
     nn = S.nn
+    B = min(args.batch_nodes if args.batch_nodes > 0 else nn, nn)
+    order = node_order(info["n"])
+    N = (2 * info["nloc"] + 1) ** 3
+    sq = SpectralQuadrature(S, precision=args.precision, batch_nodes=B)
     if args.copy_calib_mib > 0:
         calibration_copy(args.copy_calib_mib * (1 << 20))
-    for i in range(args.warmup):
-        log(f"bench: warmup {i}")
-        sq.density(veff, scf_iter=1)
-    times, last = [], None
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        last = sq.density(veff, scf_iter=1)
-        times.append(time.perf_counter() - t0)
-        log(f"bench: step {i} {times[-1]:.3f} s lambda_f={last['lambda_f']:.12f}")
-    ms = 1e3 * float(np.mean(times))
-    value = nn / (ms * 1e-3)
-    st = last["stats"]
-    # dominant z-march kernel: HIP events around every launch of the last step
-    # hot kernels, timed with HIP events around every launch of the last step
-    kt = [sq.kernel_timing(w) for w in range(4)]
-    keys = ["k_zmarch<CHEB>", "k_zmarch<LANCZOS>", "k_projs<LANCZOS>", "k_projs<CHEB>"]
-    what = ["Chebyshev recurrence: read t_k, t_{k-1}, write t_{k+1}",
-            "Lanczos pass: read r', v_{k-1}, write v_k, r'_{k+1}",
-            "projections of the Lanczos vector: read r', v_{k-1}",
-            "projections of t_k: read t_k"]
-    nvec = [3, 4, 2, 1]
-    dom = int(np.argmax([k["total_ms"] for k in kt]))
-    rs = 8 if args.precision == "fp64" else 4
-    N = (2 * info["nloc"] + 1) ** 3
-    traffic = profiled_traffic(args.workload, args.precision)
-    rl = {}
-    for w in range(4):
-        k = kt[w]
-        avg_ms = k["total_ms"] / max(k["launches"], 1)
-        alg = k["alg_bytes"] / max(k["launches"], 1)
-        rl[w] = {"kernel": f"{keys[w]} ({what[w]})", "launches": k["launches"], "avg_ms": avg_ms,
-                 "alg_bytes_per_launch": alg, "achieved_GBs": alg / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0,
-                 "alg_bytes_per_unit": f"{nvec[w]} * {N} * {rs} B per node-step",
-                 "traffic": traffic.get(keys[w])}
-    roof = {"bound": "hbm", "kernel": rl[dom]["kernel"], "achieved": rl[dom]["achieved_GBs"],
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rl[dom]["achieved_GBs"] / HBM_PEAK_GBS,
-            "traffic": rl[dom]["traffic"],
-            "alg_bytes_per_unit": rl[dom]["alg_bytes_per_unit"],
-            "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, scaled by a plain copy "
-                              "(profiles/traffic_lih216.md)" if rl[dom]["traffic"] else None,
-            "others": [rl[w] for w in range(4) if w != dom]}
+    sec, per, last = batch_steps(sq, veff, order, B, args.warmup, args.steps, args.precision)
+    value = per / sec
     out = {"metric": METRIC, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64",
            "data": "synthetic (seeded atomic positions; smooth model Veff)",
            "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "nodes": nn,
                       "nloc": info["nloc"], "npl": info["npl"], "precision": args.precision,
-                      "batch_nodes": args.batch_nodes or nn},
-           "phases_s": {"lanczos": st["t_lanczos"], "cheb": st["t_cheb"], "fermi": st["t_fermi"],
-                        "rho": st["t_rho"]},
-           "lanczos_iters": {"node0": st["lanczos_iters_node0"], "min": st["lanczos_iters_min"],
-                             "max": st["lanczos_iters_max"], "passes": st["lanczos_passes"]},
-           "roofline": roof}
-    if not args.no_mixed and args.precision == "fp64":
-        log("bench: mixed precision step")
-        sqm = SpectralQuadrature(S, precision="mixed", batch_nodes=args.batch_nodes)
-        sqm.density(veff, scf_iter=1)
-        t0 = time.perf_counter()
-        mo = sqm.density(veff, scf_iter=1)
-        tm = time.perf_counter() - t0
-        err = float(np.max(np.abs(mo["rho"] - last["rho"])) / np.max(np.abs(last["rho"])))
-        sqm.close()
-        # accuracy of the mixed arithmetic itself: both Fermi levels solved to
-        # 1e-12 (at fermi_tol = 1e-6 the two roots may differ by ~1e-6, which
-        # alone moves rho by a few 1e-6)
-        log("bench: mixed vs fp64 accuracy at fermi_tol 1e-12")
-        r64 = SpectralQuadrature(S, precision="fp64", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
-        t64 = r64.density(veff, scf_iter=1)
-        f64 = r64.ForcesNonLocal()
-        r64.close()
-        r32 = SpectralQuadrature(S, precision="mixed", batch_nodes=args.batch_nodes, fermi_tol=1e-12)
-        t32 = r32.density(veff, scf_iter=1)
-        f32 = r32.ForcesNonLocal()
-        r32.close()
-        err_t = float(np.max(np.abs(t32["rho"] - t64["rho"])) / np.max(np.abs(t64["rho"])))
-        out["mixed"] = {"value": nn / tm, "unit": "nodes/s", "ms_per_step": 1e3 * tm,
-                        "speedup_vs_fp64": (ms * 1e-3) / tm,
-                        "rho_rel_err_vs_fp64": err_t,
-                        "lambda_f_err": abs(t32["lambda_f"] - t64["lambda_f"]),
-                        "rho_rel_err_at_fermi_tol_1e-6": err,
-                        "ebs_rel_err": abs(t32["ebs"] - t64["ebs"]) / abs(t64["ebs"]),
-                        "nonlocal_force_max_err": float(np.max(np.abs(f32 - f64))),
-                        "nonlocal_force_max": float(np.max(np.abs(f64)))}
+                      "batch_nodes": B, "nodes_per_step": per,
+                      "step": "Lanczos bounds + Chebyshev moments of one batch (sqg_node_moments)"},
+           "roofline": roofline(sq, args.workload, args.precision, N)}
     sq.close()
+    if not args.no_mixed and args.precision == "fp64":
+        log("bench: mixed precision batch steps")
+        sqm = SpectralQuadrature(S, precision="mixed", batch_nodes=B)
+        msec, mper, _ = batch_steps(sqm, veff, order, B, 1, 4, "mixed")
+        out["mixed"] = {"value": mper / msec, "unit": "nodes/s", "ms_per_step": 1e3 * msec,
+                        "speedup_vs_fp64": sec / msec,
+                        "roofline": roofline(sqm, args.workload, "mixed", N)}
+        sqm.close()
+    if not args.no_full:
+        # the whole density step (every batch, global bounds, Fermi level, rho,
+        # Ebs, Eent); both Fermi levels solved to 1e-12 so that the density
+        # difference is the mixed arithmetic's own (at the reference's 1e-6 the
+        # two roots alone may differ by ~1e-6)
+        log("bench: full fp64 density step")
+        r64 = SpectralQuadrature(S, precision="fp64", batch_nodes=B, fermi_tol=1e-12)
+        t0 = time.perf_counter()
+        d64 = r64.density(veff, scf_iter=1)
+        t64 = time.perf_counter() - t0
+        st = d64["stats"]
+        r64.close()
+        out["full_step"] = {"s": t64, "nodes_per_s": nn / t64, "precision": "fp64", "fermi_tol": 1e-12,
+                            "lambda_f": d64["lambda_f"], "electrons": float(d64["rho"].sum() * info["h"] ** 3),
+                            "phases_s": {"lanczos": st["t_lanczos"], "cheb": st["t_cheb"],
+                                         "fermi": st["t_fermi"], "rho": st["t_rho"]},
+                            "lanczos_iters": {"node0": st["lanczos_iters_node0"], "min": st["lanczos_iters_min"],
+                                              "max": st["lanczos_iters_max"], "passes": st["lanczos_passes"]},
+                            "brent_iters": st["fermi_iters"]}
+        if not args.no_mixed:
+            log("bench: full mixed density step")
+            r32 = SpectralQuadrature(S, precision="mixed", batch_nodes=B, fermi_tol=1e-12)
+            t0 = time.perf_counter()
+            d32 = r32.density(veff, scf_iter=1)
+            t32 = time.perf_counter() - t0
+            r32.close()
+            out["full_step"]["mixed"] = {
+                "s": t32, "nodes_per_s": nn / t32, "speedup_vs_fp64": t64 / t32,
+                "rho_rel_err_vs_fp64": float(np.max(np.abs(d32["rho"] - d64["rho"])) / np.max(np.abs(d64["rho"]))),
+                "lambda_f_err": abs(d32["lambda_f"] - d64["lambda_f"]),
+                "ebs_rel_err": abs(d32["ebs"] - d64["ebs"]) / abs(d64["ebs"])}
     if not args.no_scf:
-        # one SCF iteration on the workload's own pseudocharge: SQ density,
-        # total energy, AAJ Poisson from the previous phi, LDA potential and the
-        # Veff residual (src/scf.cpp:145-200); the second iteration is timed
-        log("bench: scf iterations")
+        # one SCF iteration of the 216-atom LiH cell on its own pseudocharge:
+        # SQ density, total energy, AAJ Poisson from the previous phi, LDA
+        # potential and the Veff residual (src/scf.cpp:145-200); the second
+        # iteration is timed, then the forces at that state, FP64 and mixed
+        log("bench: lih216 scf iterations")
         from sqdft_amd.scf import SCF
-        scf = SCF(S, precision=args.precision, batch_nodes=args.batch_nodes)
+        L = configs.build("lih216", seed=0)
+        scf = SCF(L, precision="fp64")
         t0 = time.perf_counter()
         scf.Pseudocharge()
         t_pc = time.perf_counter() - t0
@@ -261,25 +290,32 @@ def main():
         it1 = scf.iteration()
         scf.veff = it1["veff_new"]
         it2 = scf.iteration()
-        log("bench: forces")
+        log("bench: lih216 forces")
         t0 = time.perf_counter()
         scf.ForcesLocal()
         t_loc = time.perf_counter() - t0
         t0 = time.perf_counter()
-        scf.sq.ForcesNonLocal()
+        fnl = scf.sq.ForcesNonLocal()
         t_nl = time.perf_counter() - t0
+        v2 = scf.veff
         scf.close()
-        out["scf_step"] = {"s": it2["t_sq"] + it2["t_rest"], "sq_s": it2["t_sq"],
+        log("bench: lih216 mixed nonlocal forces at the same Veff")
+        sqm = SpectralQuadrature(L, precision="mixed")
+        sqm.density(v2, scf_iter=2, lambda_f=it2["lambda_f"])
+        fnl32 = sqm.ForcesNonLocal()
+        sqm.close()
+        out["scf_step"] = {"workload": "lih216: " + WORKLOADS["lih216"],
+                           "s": it2["t_sq"] + it2["t_rest"], "sq_s": it2["t_sq"],
                            "energy_poisson_vxc_s": it2["t_rest"], "poisson_iters": it2["poisson"][0],
                            "scf_err": it2["scf_err"], "lambda_f": it2["lambda_f"],
                            "forces_local_s": t_loc, "forces_nonlocal_s": t_nl, "pseudocharge_s": t_pc,
+                           "mixed_nonlocal_force_max_err": float(np.max(np.abs(fnl32 - fnl))),
+                           "nonlocal_force_max": float(np.max(np.abs(fnl))),
                            "what": "second SCF iteration from the atomic starting density, unmixed Veff; "
                                    "then local and nonlocal forces at that state"}
-    if not args.no_scf:
         # SCF iterations to SCF_err 1e-6 on the reference's Al test input:
         # its Periodic Pulay mixing vs Newton steps from the adjoint response
         log("bench: scf mixing on the Al test input")
-        from sqdft_amd.scf import SCF
         from sqdft_amd.system import System
         A = System.from_files(os.path.join(ROOT, "tests", "golden", "Al"), "test_Al")
         mix = {}
@@ -291,6 +327,8 @@ def main():
                          "response_products": r["response_products"], "s": time.perf_counter() - t0,
                          "free_energy": r["free_energy"]}
             sc.close()
+        out["scf_mixing"] = {"workload": "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6",
+                             **mix}
         log("bench: NVE steps on the Al test input")
         from sqdft_amd.md import NVE
         sc = SCF(A)
@@ -302,8 +340,6 @@ def main():
                      "s_per_step": [r["seconds"] for r in recs], "scf_iterations": [r["scf_iterations"] for r in recs],
                      "TE": [r["TE"] for r in recs]}
         A.close()
-        out["scf_mixing"] = {"workload": "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6",
-                             **mix}
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(S, veff, cfg["Rcut"], cfg["T"], args.cpu_seconds)

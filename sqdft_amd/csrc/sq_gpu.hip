@@ -74,9 +74,10 @@ struct sqg_ctx {
     DBuf Dm, contrib, ncontrib;   // nonlocal forces
     // batch buffers
     int Bcap = 0;
-    DBuf X, Y, NLv, s, node_list, list, list_cnt, flags;
+    DBuf X, Y, X2, Y2, NLv, s, node_list, list, list_cnt, flags;   // X2, Y2: Lanczos ping-pong
     DBuf glist, glist_cnt;         // group image lists (NG slots per group)
     int gl_max = 0;                // longest group list of the current batch
+    bool projr_ok = false;         // k_projr's slot plan holds for every group of the batch
     DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, cnt, done, ca, ib, bprev;
     DBuf chiB, zeeB, center, dots, htemp;
     // all-node state of the last density step
@@ -134,24 +135,9 @@ Geo make_geo(const sqg_ctx *c, const int *dnodes, const int *ddone)
     return g;
 }
 
-// Shape of the z-march workgroup: columns per thread (CPT) and launch bound
-// (LB).  A plane of M <= 32 fits 1024 threads at one column per thread;
-// larger cubes take two columns (M <= 35 within 640 threads) or more.  FP32
-// Chebyshev steps take two columns in 512 threads from M <= 32 on as well
-// (measured 13 % faster; FP32 Lanczos passes are faster at one column).
-struct ZShape { int cpt, lb; };
-int zmarch_threads_cpt(int M, int cpt) { return (((M + cpt - 1) / cpt * M) + 63) / 64 * 64; }
-ZShape zmarch_shape(int M, bool fp64, int mode)
-{
-    if (!fp64 && mode == MODE_CHEB && zmarch_threads_cpt(M, 2) <= 512) return {2, 512};
-    if (zmarch_threads_cpt(M, 1) <= 1024) return {1, 1024};
-    if (zmarch_threads_cpt(M, 2) <= 640) return {2, 640};
-    if (zmarch_threads_cpt(M, 2) <= 1024 && !fp64) return {2, 1024};
-    for (int cpt = 3; cpt < 4; ++cpt)
-        if (zmarch_threads_cpt(M, cpt) <= 512) return {cpt, 512};
-    return {4, 512};   // M <= 43 (MAXM) fits 512 threads at four columns
-}
-int zmarch_threads(int M, bool fp64, int mode) { return zmarch_threads_cpt(M, zmarch_shape(M, fp64, mode).cpt); }
+// Threads of one z-march workgroup: one column per thread over the larger
+// half of the cube's rows (two workgroups per node cube).
+int zmarch_threads(int M) { return (zhalf_rows(M, 0) * M + 63) / 64 * 64; }
 
 // Visiting order of the FD nodes: 4x4x2 bricks, so that workgroups that run
 // at the same time (consecutive batch slots) march overlapping cubes and
@@ -178,6 +164,8 @@ int ensure_batch(sqg_ctx *c, int B)
     const size_t S = MAXIT + 2;
     HIPCHK(c->X.ensure((size_t)B * c->M3 * rs));
     HIPCHK(c->Y.ensure((size_t)B * c->M3 * rs));
+    HIPCHK(c->X2.ensure((size_t)B * c->M3 * rs));
+    HIPCHK(c->Y2.ensure((size_t)B * c->M3 * rs));
     HIPCHK(c->NLv.ensure((size_t)B * c->M3 * rs));
     HIPCHK(c->s.ensure((size_t)B * std::max(c->n_ip, 1) * sizeof(double)));
     HIPCHK(c->node_list.ensure((size_t)B * sizeof(int)));
@@ -186,12 +174,12 @@ int ensure_batch(sqg_ctx *c, int B)
     HIPCHK(c->glist.ensure((size_t)((B + NG - 1) / NG) * GLCAP * sizeof(int)));
     HIPCHK(c->glist_cnt.ensure((size_t)((B + NG - 1) / NG) * sizeof(int)));
     HIPCHK(c->flags.ensure(16 * sizeof(int)));
-    HIPCHK(c->lred.ensure((size_t)B * 4 * sizeof(double)));
+    HIPCHK(c->lred.ensure((size_t)B * ZHALF * 4 * sizeof(double)));
     for (DBuf *d : {&c->aa, &c->bb, &c->dq, &c->eq}) HIPCHK(d->ensure((size_t)B * S * sizeof(double)));
     for (DBuf *d : {&c->lmin, &c->lmax, &c->lmin_prev, &c->lmax_prev, &c->ca, &c->ib, &c->bprev,
                     &c->chiB, &c->zeeB, &c->center, &c->htemp})
         HIPCHK(d->ensure((size_t)B * sizeof(double)));
-    HIPCHK(c->dots.ensure((size_t)B * 2 * sizeof(double)));
+    HIPCHK(c->dots.ensure((size_t)B * ZHALF * 2 * sizeof(double)));
     HIPCHK(c->cnt.ensure((size_t)B * sizeof(int)));
     HIPCHK(c->done.ensure((size_t)B * sizeof(int)));
     c->Bcap = B;
@@ -204,7 +192,7 @@ int choose_batch(sqg_ctx *c, int want, int nvec = 2)
     B = std::min(B, want);
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) == hipSuccess && freeb > 0) {
-        const size_t per = (nvec + 1) * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
+        const size_t per = (nvec + 3) * (size_t)c->M3 * c->real_size() + (size_t)std::max(c->n_ip, 1) * 8 +
                            LCAP * 4 + GLCAP * 4 / NG + 4 * (MAXIT + 2) * 8 + 256;
         const size_t cap = (size_t)(0.6 * (double)(freeb + (size_t)c->Bcap * per)) / per;
         if ((size_t)B > cap) B = (int)std::max<size_t>(cap, 1);
@@ -263,8 +251,27 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     return A;
 }
 
+// Shape of k_projr (s-channel tables only): members per workgroup, register
+// slots per wave and planes per slab; nm = 0 when it does not apply.
+constexpr int PROJR_NT = 512, PROJR_RPI = 3;
+struct ProjrShape { int nm, R, sz; };
+ProjrShape projr_shape(const sqg_ctx *c)
+{
+    if (c->max_nproj != 1 || c->n_ip == 0) return {0, 0, 0};
+    int w = 0;
+    for (int t = 0; t < c->n_typ; ++t)
+        if (c->h_nproj[t] > 0) w = std::max(w, 2 * c->h_nlocc[t] + 1);
+    if (w * w > 64 * PROJR_RPI) return {0, 0, 0};
+    ProjrShape s{8, 6, 1};
+    if (const char *e = std::getenv("SQG_PROJR")) {   // experiment: "<nm>:<R>:<sz>" or "off"
+        if (std::strcmp(e, "off") == 0) return {0, 0, 0};
+        std::sscanf(e, "%d:%d:%d", &s.nm, &s.R, &s.sz);
+    }
+    return s;
+}
+
 // Image lists of the B nodes in node_list (per node and per group of NG
-// slots); sets gl_max.  Overflow goes to flags[0].
+// slots); sets gl_max and projr_ok.  Overflow goes to flags[0].
 int node_images(sqg_ctx *c, int B)
 {
     LArgs L{};
@@ -279,14 +286,22 @@ int node_images(sqg_ctx *c, int B)
     L.overflow = c->flags.as<int>();
     L.glist = c->glist.as<int>();
     L.glist_cnt = c->glist_cnt.as<int>();
+    const ProjrShape ps = projr_shape(c);
+    HIPCHK(hipMemsetAsync(c->flags.as<int>() + 4, 0, sizeof(int), c->st));
+    if (ps.nm) {
+        L.pr_bad = c->flags.as<int>() + 4;
+        L.pr_R = ps.R; L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz;
+    }
     hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
     const int ng = (B + NG - 1) / NG;
     hipLaunchKernelGGL(k_group_images, dim3((ng + 63) / 64), dim3(64), 0, c->st, L, B);
-    std::vector<int> gc(ng);
+    std::vector<int> gc(ng + 1);
     HIPCHK(hipMemcpyAsync(gc.data(), c->glist_cnt.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(gc.data() + ng, c->flags.as<int>() + 4, sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     c->gl_max = 0;
-    for (int v : gc) c->gl_max = std::max(c->gl_max, v);
+    for (int i = 0; i < ng; ++i) c->gl_max = std::max(c->gl_max, gc[i]);
+    c->projr_ok = ps.nm != 0 && gc[ng] == 0;
     return SQG_OK;
 }
 
@@ -318,43 +333,95 @@ void nlapply_go(sqg_ctx *c, int B, const NArgs<T> &A)
     hipLaunchKernelGGL(fn, dim3(B), dim3(512), shm, c->st, A);
 }
 
-template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU>
+template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU, int NM>
 void projs_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
 {
-    auto fn = k_projs<T, MAXP, FORMED, SZ, NT, NPU>;
+    auto fn = k_projs<T, MAXP, FORMED, SZ, NT, NPU, NM>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(fn, grid, dim3(NT), shm, c->st, P);
+    hipLaunchKernelGGL(fn, dim3(grid.x * (NG / NM)), dim3(NT), shm, c->st, P);
 }
 
-// Staged projections with the deepest slab whose planes fit in LDS; false
-// when not even a single cube plane fits.
+template <typename T, int FORMED, int SZ, int NM>
+void projs_np(sqg_ctx *c, dim3 grid, size_t shs, const PArgs<T> &P, int npu)
+{
+    constexpr int NT = 512;
+    if (npu == 1) projs_go<T, 1, FORMED, SZ, NT, 1, NM>(c, grid, shs, P);
+    else if (npu == 3) projs_go<T, 4, FORMED, SZ, NT, 3, NM>(c, grid, shs, P);
+    else projs_go<T, 4, FORMED, SZ, NT, 4, NM>(c, grid, shs, P);
+}
+
+// Staged projections: members per workgroup (NM), planes per slab (sz) and
+// the largest union plane (pu) staged at once.  Returns false when not even
+// a single cube plane fits in LDS.
 template <typename T, int FORMED>
 bool launch_projs(sqg_ctx *c, dim3 grid, PArgs<T> &P, int mp)
 {
     constexpr int D = sizeof(T) == 4 ? 4 : 2;   // deep slab (planes)
-    constexpr int H = D / 2;
-    constexpr int NT = 512;
     const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
-    for (int pu : {(c->M + 3) * (c->M + 3), c->M2}) {
-        for (int sz : {D, H}) {
-            const size_t shs = sa + (size_t)pu * NG * sizeof(T) * sz;
-            if (shs > 160 * 1024) continue;
-            P.pu = pu;
-            // projector values are loaded and summed for npu of the mp slots
-            const int npu = mp == 1 ? 1 : c->max_nproj == 3 ? 3 : 4;
-            if (sz == D) {
-                if (npu == 1) projs_go<T, 1, FORMED, D, NT, 1>(c, grid, shs, P);
-                else if (npu == 3) projs_go<T, 4, FORMED, D, NT, 3>(c, grid, shs, P);
-                else projs_go<T, 4, FORMED, D, NT, 4>(c, grid, shs, P);
-            } else {
-                if (npu == 1) projs_go<T, 1, FORMED, H, NT, 1>(c, grid, shs, P);
-                else if (npu == 3) projs_go<T, 4, FORMED, H, NT, 3>(c, grid, shs, P);
-                else projs_go<T, 4, FORMED, H, NT, 4>(c, grid, shs, P);
+    // projector values are loaded and summed for npu of the mp slots
+    const int npu = mp == 1 ? 1 : c->max_nproj == 3 ? 3 : 4;
+    int want_nm = 8, want_sz = 0;
+    if (const char *e = std::getenv("SQG_PROJ_VARIANT")) {   // experiment: "<nm>:<sz>"
+        want_nm = std::atoi(e);
+        const char *k = std::strchr(e, ':');
+        if (k) want_sz = std::atoi(k + 1);
+    }
+    for (int nm : {want_nm, 8}) {
+        const int pus[2] = {nm == 8 ? (c->M + 3) * (c->M + 3) : (c->M + 1) * (c->M + 1), c->M2};
+        for (int pu : pus) {
+            for (int sz : {D, D / 2, 1}) {
+                if (want_sz && sz != want_sz && nm == want_nm) continue;
+                const size_t shs = sa + (size_t)pu * nm * sizeof(T) * sz;
+                if (shs > 160 * 1024) continue;
+                P.pu = pu;
+                if (nm == 4) {
+                    if (sz == 4) projs_np<T, FORMED, 4, 4>(c, grid, shs, P, npu);
+                    else if (sz == 2) projs_np<T, FORMED, 2, 4>(c, grid, shs, P, npu);
+                    else projs_np<T, FORMED, 1, 4>(c, grid, shs, P, npu);
+                } else {
+                    if (sz == 4) projs_np<T, FORMED, 4, 8>(c, grid, shs, P, npu);
+                    else if (sz == 2) projs_np<T, FORMED, 2, 8>(c, grid, shs, P, npu);
+                    else projs_np<T, FORMED, 1, 8>(c, grid, shs, P, npu);
+                }
+                return true;
             }
-            return true;
         }
     }
     return false;
+}
+
+template <typename T, int FORMED, int SZ, int NM, int R>
+void projr_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
+{
+    auto fn = k_projr<T, FORMED, SZ, PROJR_NT, NM, R, PROJR_RPI>;
+    if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    hipLaunchKernelGGL(fn, dim3(grid.x * (NG / NM)), dim3(PROJR_NT), shm, c->st, P);
+}
+
+template <typename T, int FORMED>
+bool launch_projr(sqg_ctx *c, dim3 grid, PArgs<T> &P)
+{
+    const ProjrShape ps = projr_shape(c);
+    const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
+    // slab capacity per plane and member: the union plane of a 4x2 (NM = 8)
+    // or 2x2 (NM = 4) block of cubes, or one cube, with rows padded to a pitch
+    // congruent to the box side modulo 32 (as k_projr pads them)
+    int wbox = 0;
+    for (int t = 0; t < c->n_typ; ++t)
+        if (c->h_nproj[t] > 0) wbox = 2 * c->h_nlocc[t] + 1;
+    auto upitch = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
+    const int ux = ps.nm == 8 ? c->M + 3 : c->M + 1, uy = c->M + 1;
+    const int pu = std::max(upitch(ux) * uy, upitch(c->M) * c->M);
+    const size_t shm = sa + (size_t)pu * ps.nm * sizeof(T) * ps.sz;
+    if (shm > 160 * 1024) return false;
+    P.pu = pu;
+    if (ps.nm == 8 && ps.R == 5 && ps.sz == 1) projr_go<T, FORMED, 1, 8, 5>(c, grid, shm, P);
+    else if (ps.nm == 8 && ps.R == 6 && ps.sz == 1) projr_go<T, FORMED, 1, 8, 6>(c, grid, shm, P);
+    else if (ps.nm == 8 && ps.R == 6 && ps.sz == 2) projr_go<T, FORMED, 2, 8, 6>(c, grid, shm, P);
+    else if (ps.nm == 4 && ps.R == 8 && ps.sz == 1) projr_go<T, FORMED, 1, 4, 8>(c, grid, shm, P);
+    else if (ps.nm == 4 && ps.R == 8 && ps.sz == 2) projr_go<T, FORMED, 2, 4, 8>(c, grid, shm, P);
+    else return false;
+    return true;
 }
 
 // Projections s = <U, v>/Denom of the B vectors (k_proj), then their
@@ -372,6 +439,7 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A, int which = -1, double al
         PArgs<T> P = A;
         P.B = B;
         const dim3 grid((B + NG - 1) / NG);
+        if (c->projr_ok && launch_projr<T, FORMED>(c, grid, P)) goto apply;
         // staged projections (k_projs) with the deepest slab whose planes
         // (a union of a 4x4 footprint, else a single cube plane) fit in LDS
         const int mp = c->max_nproj <= 1 ? 1 : c->max_nproj <= 4 ? 4 : 0;
@@ -416,12 +484,12 @@ size_t zmarch_lds(const sqg_ctx *c)
     return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double);
 }
 
-template <typename T, int MODE, int CPT, int LB>
+template <typename T, int MODE, int LB>
 void zmarch_k(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
 {
-    auto fn = k_zmarch<T, MODE, CPT, LB>;
+    auto fn = LB <= 640 ? k_zmarch<T, MODE> : k_zmarch_big<T, MODE>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-    hipLaunchKernelGGL(fn, dim3(B), dim3(zmarch_threads(c->M, sizeof(T) == 8, MODE)), shm, c->st, A);
+    hipLaunchKernelGGL(fn, dim3(B * ZHALF), dim3(zmarch_threads(c->M)), shm, c->st, A);
 }
 
 template <typename T, int MODE>
@@ -434,13 +502,10 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         e1 = c->ev_get();
         if (e0) (void)hipEventRecord(e0, c->st);
     }
-    const ZShape zs = zmarch_shape(c->M, sizeof(T) == 8, MODE);
-    if (zs.cpt == 1) zmarch_k<T, MODE, 1, 1024>(c, B, A, shm);
-    else if (zs.cpt == 2 && zs.lb == 512) zmarch_k<T, MODE, 2, 512>(c, B, A, shm);
-    else if (zs.cpt == 2 && zs.lb == 640) zmarch_k<T, MODE, 2, 640>(c, B, A, shm);
-    else if (zs.cpt == 2) zmarch_k<T, MODE, 2, 1024>(c, B, A, shm);
-    else if (zs.cpt == 3) zmarch_k<T, MODE, 3, 512>(c, B, A, shm);
-    else zmarch_k<T, MODE, 4, 512>(c, B, A, shm);
+    // FP32 workgroups of up to 640 threads (M <= 35) are held to 96 VGPRs so
+    // that two share a CU; FP64 ones would spill there and run one per CU
+    if (sizeof(T) == 4 && zmarch_threads(c->M) <= 640 && !std::getenv("SQG_ZBIG")) zmarch_k<T, MODE, 640>(c, B, A, shm);
+    else zmarch_k<T, MODE, 1024>(c, B, A, shm);
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
         c->ev[which].push_back({e0, e1, alg_bytes});
@@ -488,12 +553,16 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     std::vector<int> hdone(B);
     int active = B;
     long passes = 0;
+    T *X2 = c->X2.as<T>(), *Y2 = c->Y2.as<T>();
     for (int pass = 0; pass <= MAXIT && active > 0; ++pass) {
-        PArgs<T> P = pargs<T>(c, dn, ddone, X);
-        P.V2 = Y;
+        // (r', v_{k-1}) in one pair of buffers, (r'_{k+1}, v_k) out into the other
+        T *Xi = (pass & 1) ? X2 : X, *Yi = (pass & 1) ? Y2 : Y;
+        T *Xo = (pass & 1) ? X : X2, *Yo = (pass & 1) ? Y : Y2;
+        PArgs<T> P = pargs<T>(c, dn, ddone, Xi);
+        P.V2 = Yi;
         launch_proj<T, 1>(c, B, P, 2, 2.0 * active * c->M3 * rs);   // reads r' and v_{k-1}
         ZArgs<T> Z = zargs<T>(c, dn, ddone);
-        Z.X = X; Z.Y = Y;
+        Z.X = Xi; Z.Y = Yi; Z.XO = Xo; Z.YO = Yo;
         Z.ca = c->ca.as<double>(); Z.ib = c->ib.as<double>(); Z.bprev = c->bprev.as<double>();
         Z.lred = c->lred.as<double>();
         // reads r' and v_{k-1}, writes v_k and r'_{k+1}
@@ -578,6 +647,25 @@ __global__ void k_scatter_batch(int B, const int *node_list, const double *chiB,
     if (b >= B) return;
     const int nd = node_list[b];
     chi[nd] = chiB[b]; zee[nd] = zeeB[b]; lmin[nd] = lmn[b]; lmax[nd] = lmx[b]; iters[nd] = cnt[b];
+}
+
+// Per-node results of a node list, gathered from the all-node state
+// (iteration counts as doubles, exact).
+__global__ void k_gather_nodes(int nb, int K, const int *nodes, const double *chi, const double *zee,
+                               const int *iters, const double *mu, double *ochi, double *ozee, double *oit,
+                               double *omu)
+{
+    const size_t tot = (size_t)nb * K;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i / K), k = (int)(i - (size_t)b * K);
+        const int nd = nodes[b];
+        omu[i] = mu[(size_t)nd * K + k];
+        if (k == 0) {
+            ochi[b] = chi[nd];
+            ozee[b] = zee[nd];
+            oit[b] = (double)iters[nd];
+        }
+    }
 }
 
 template <typename T>
@@ -891,8 +979,11 @@ int sqg_create(const sqg_params *p, sqg_ctx **out)
     c->M3 = c->M2 * c->M;
     c->E = p->n + 2 * p->nloc;
     c->nn = p->n * p->n * p->n;
-    c->pitch = (c->M + 2 * FDN + 1) / 2 * 2;
-    c->plane = c->pitch * (c->M + 2 * FDN);
+    // LDS row pitch of the z-march plane: at least M + 12 (six zero columns
+    // each side) and congruent to M modulo 32 doubles, so that the two row
+    // pieces one wave covers fall on disjoint LDS banks
+    c->pitch = c->M + 32;
+    c->plane = c->pitch * (zhalf_rows(c->M, 0) + 2 * FDN);   // a half's rows and six beyond each side
     // Chebyshev-Gauss nodes and cosine table with the reference's expressions (sq.cpp:1241-1257)
     {
         const int K = p->npl + 1;
@@ -1060,19 +1151,26 @@ int sqg_node_moments(sqg_ctx *c, const double *veff, const int *nodes, int nb, d
     st = moments_dispatch(c, nodes, nb, &passes);
     if (st) return st;
     timing_sum(c);
+    if (nb == 0) return SQG_OK;
+    // gather the listed nodes' results on the device, then one copy each
     const size_t K = (size_t)c->p.npl + 1;
-    std::vector<double> a(c->nn), z(c->nn);
-    std::vector<int> it(c->nn);
-    HIPCHK(hipMemcpy(a.data(), c->chi_all.p, c->nn * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(z.data(), c->zee_all.p, c->nn * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(it.data(), c->iters_all.p, c->nn * sizeof(int), hipMemcpyDeviceToHost));
+    DBuf dn, g;
+    HIPCHK(dn.ensure((size_t)nb * sizeof(int)));
+    HIPCHK(g.ensure((size_t)nb * (K + 3) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(dn.p, nodes, (size_t)nb * sizeof(int), hipMemcpyHostToDevice, c->st));
+    double *gd = g.as<double>();
+    hipLaunchKernelGGL(k_gather_nodes, dim3(grid1((size_t)nb * K, 256)), dim3(256), 0, c->st, nb, (int)K,
+                       dn.as<int>(), c->chi_all.as<double>(), c->zee_all.as<double>(), c->iters_all.as<int>(),
+                       c->mu_all.as<double>(), gd, gd + nb, gd + 2 * (size_t)nb, gd + 3 * (size_t)nb);
+    std::vector<double> h((size_t)nb * (K + 3));
+    HIPCHK(hipMemcpyAsync(h.data(), gd, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     for (int i = 0; i < nb; ++i) {
-        if (chi) chi[i] = a[nodes[i]];
-        if (zee) zee[i] = z[nodes[i]];
-        if (iters) iters[i] = it[nodes[i]];
-        if (mu) HIPCHK(hipMemcpy(mu + i * K, c->mu_all.as<double>() + (size_t)nodes[i] * K, K * sizeof(double),
-                                 hipMemcpyDeviceToHost));
+        if (chi) chi[i] = h[i];
+        if (zee) zee[i] = h[nb + i];
+        if (iters) iters[i] = (int)h[2 * (size_t)nb + i];
     }
+    if (mu) std::memcpy(mu, h.data() + 3 * (size_t)nb, (size_t)nb * K * sizeof(double));
     return SQG_OK;
 }
 

@@ -114,15 +114,17 @@ struct ZArgs {
     T *X;                         // input (HSUB/CHEB: t_k; LANCZOS: r')
     T *Y;                         // HSUB: out; CHEB: t_{k-1} in / t_{k+1} out; LANCZOS: v
     T *O;                         // CHEB: t_{k+1} goes here instead of Y when set
+    T *XO, *YO;                   // LANCZOS: r'_{k+1} and v_k out (the other pair of
+                                  // buffers: the other half-workgroup reads X, Y as halo)
     // CHEB per node
     const double *chi, *zee;      // indexed by batch slot, or by node id if chi_by_node
     int chi_by_node;
     int first;                    // CHEB: t1 = (H t0 - chi t0)/zee, no t_{k-1}
     double *center;               // [B] t_{k+1}[c]
-    double *dots;                 // [B][2] (t_{k+1}.t_{k+1}, t_{k+1}.t_k)
+    double *dots;                 // [B][2 halves][2] (t_{k+1}.t_{k+1}, t_{k+1}.t_k)
     // LANCZOS per node
     const double *ca, *ib, *bprev;  // [B] alpha_{k-1}, 1/beta_k, beta_k
-    double *lred;                 // [B][4] v.w, r.r, r.v, v.v
+    double *lred;                 // [B][2 halves][4] v.w, r.r, r.v, v.v
     // CHEBD: CHEB with a density-matrix column (forces.cpp:410-460):
     // D += Ci[nd][k] * t_k / h^3 for the produced t_k
     T *D;                         // [B][M^3]
@@ -223,37 +225,46 @@ __global__ void __launch_bounds__(512) k_nlapply(NArgs<T> A)
     }
 }
 
-// One workgroup per node cube.  Thread (px, y) owns the CPT columns
-// x = CPT px + c of row y for every z.  The 13 z-neighbours of each column
+// Two workgroups per node cube, one per half of its rows: half h owns rows
+// [y0, y1) (y0 = 0, y1 = ceil(M/2) for h = 0; the rest for h = 1) and thread
+// (x, y) one column of them for every z.  The 13 z-neighbours of each column
 // live in a register window that shifts by one plane per step; the current
-// plane is staged in LDS for the x/y-neighbours.  The pointwise inputs of
-// plane z (Veff, the nonlocal term, t_{k-1} or v_{k-1}, D) are loaded during
-// step z - 1 into one of two register sets that alternate between the steps
-// of the loop (unrolled by two, so no copy waits on a load in flight); the
-// window feed of plane z + 7 is issued at the start of step z and inserted
-// at its end.  The per-plane barrier orders LDS traffic only.
-template <typename T, int MODE, int CPT, int LB>
-__global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
+// plane is staged in LDS for the x/y-neighbours, together with the six rows
+// beyond the half (loaded as plain input values, no window: only the y-stencil
+// reads them).  The pointwise inputs of plane z (Veff, the nonlocal term,
+// t_{k-1} or v_{k-1}, D, the halo rows) are loaded during step z - 1 into one
+// of two register sets that alternate between the steps of the loop (unrolled
+// by two, so no copy waits on a load in flight); the window feed of plane
+// z + 7 is issued at the start of step z and inserted at its end.  The
+// per-plane barrier orders LDS traffic only.  Each half reduces its own dot
+// products into its slot of lred / dots ([B][2][.]); the half that holds the
+// cube centre writes the centre value.
+constexpr int ZHALF = 2;
+__host__ __device__ inline int zhalf_rows(int M, int h) { return h == 0 ? (M + 1) / 2 : M / 2; }
+
+template <typename T, int MODE>
+__device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
 {
     constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
-    const int b = blockIdx.x;
+    const int b = blockIdx.x / ZHALF, h = blockIdx.x % ZHALF;
     if (G.done && G.done[b]) return;
     const int node = G.node_list[b];
     const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, PP = G.pitch;
     const int ni = node % n, nj = (node / n) % n, nk = node / (n * n);
-    const int npx = (M + CPT - 1) / CPT;
+    const int y0 = h == 0 ? 0 : (M + 1) / 2, ny = zhalf_rows(M, h);
     const int tid = threadIdx.x;
-    const int py = tid / npx;
-    const int px = tid - py * npx;
-    const bool act = py < M;
-    const int x0 = CPT * px;
-    bool has[CPT];
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) has[c] = act && (x0 + c < M);
+    const int pyl = tid / M;
+    const int x0 = tid - pyl * M;
+    const int py = y0 + pyl;
+    const bool act = pyl < ny;
+    // halo point of this thread: row hy (6 rows beyond the half), column hx
+    const int hr = tid / M, hx = tid - hr * M;
+    const int hy = h == 0 ? y0 + ny + hr : y0 - FDN + hr;
+    const bool hact = hr < FDN && hy >= 0 && hy < M;
 
-    // LDS: two staged planes | reduction scratch
+    // LDS: two staged planes (rows y0 - 6 .. y0 + ny + 5) | reduction scratch
     extern __shared__ unsigned char smem_raw[];
     T *lds = reinterpret_cast<T *>(smem_raw);
     double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
@@ -273,10 +284,9 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     const __amdgpu_buffer_rsrc_t rD = brsrc(MODE == MODE_CHEBD ? Db : Yb, vb);
     const __amdgpu_buffer_rsrc_t rV = brsrc(A.veff, (size_t)n * n * n * TS);
 
-    const int col = py * M + x0;                 // offset of the first column in a plane
-    int gx[CPT];
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) gx[c] = wrapn(ni - nloc + x0 + c, n);
+    const int col = py * M + x0;                 // offset of the column in a plane
+    const int hcol = hy * M + hx;                // offset of the halo point in a plane
+    const int gx = wrapn(ni - nloc + x0, n);
     const int gyo = wrapn(nj - nloc + py, n) * n;
     // Arithmetic is FP64 in both modes: the mixed mode stores vectors, Veff
     // and the nonlocal term in FP32 (half the HBM traffic) but forms H v, the
@@ -312,111 +322,104 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     const bool cheb_prev = CH && !A.first;
     const bool dacc = MODE == MODE_CHEBD && !A.first;
 
-    // byte offset of (plane q, column c) in a node vector, or OOB
-    auto voff = [&](int q, int c) -> unsigned {
-        return (has[c] && q >= 0 && q < M) ? (unsigned)(q * M2 + col + c) * TS : OOB;
+    // byte offset of (plane q) in a node vector for this column / halo point, or OOB
+    auto voff = [&](int q) -> unsigned {
+        return (act && q >= 0 && q < M) ? (unsigned)(q * M2 + col) * TS : OOB;
+    };
+    auto hoff = [&](int q) -> unsigned {
+        return (hact && q >= 0 && q < M) ? (unsigned)(q * M2 + hcol) * TS : OOB;
     };
     // window feed: input value of plane q (LANCZOS forms v = (r' - alpha v_{k-1}) / beta)
-    auto feed = [&](int q, int c) -> T {
-        const unsigned o = voff(q, c);
+    auto feed = [&](int q) -> T {
+        const unsigned o = voff(q);
         if (MODE == MODE_LANCZOS) return (bload<T>(rX, o) - ca * bload<T>(rY, o)) * ib;
         return bload<T>(rX, o);
     };
     struct PW {
-        T ve[CPT], nl[CPT], pv[CPT], dv[CPT];   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
+        T ve, nl, pv, dv;   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
+        T hx, hyv;          // halo point: the two stored vectors
     };
     auto fill_pw = [&](int z, PW &P) {
-#pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-            const unsigned o = voff(z, c);
-            const int vo = (wrapn(nk - nloc + z, n) * n) * n + gyo + gx[c];
-            P.ve[c] = bload<T>(rV, (has[c] && z < M) ? (unsigned)vo * TS : OOB);
-            P.nl[c] = bload<T>(rN, o);
-            P.pv[c] = (cheb_prev || MODE == MODE_LANCZOS) ? bload<T>(rY, o) : (T)0;
-            P.dv[c] = dacc ? bload<T>(rD, o) : (T)0;
-        }
+        const unsigned o = voff(z);
+        const int vo = (wrapn(nk - nloc + z, n) * n) * n + gyo + gx;
+        P.ve = bload<T>(rV, (act && z < M) ? (unsigned)vo * TS : OOB);
+        P.nl = bload<T>(rN, o);
+        P.pv = (cheb_prev || MODE == MODE_LANCZOS) ? bload<T>(rY, o) : (T)0;
+        P.dv = dacc ? bload<T>(rD, o) : (T)0;
+        const unsigned ho = hoff(z);
+        P.hx = bload<T>(rX, ho);
+        P.hyv = MODE == MODE_LANCZOS ? bload<T>(rY, ho) : (T)0;
     };
 
-    // window: w[c][s] holds plane z - 6 + s during step z
-    T w[CPT][WIN];
+    // window: w[s] holds plane z - 6 + s during step z
+    T w[WIN];
     PW SA, SB;
 #pragma unroll
-    for (int c = 0; c < CPT; ++c) {
+    for (int s = 0; s < FDN; ++s) w[s] = 0;
 #pragma unroll
-        for (int s = 0; s < FDN; ++s) w[c][s] = 0;
-#pragma unroll
-        for (int s = 0; s <= FDN; ++s) w[c][FDN + s] = feed(s, c);   // planes 0..6
-    }
+    for (int s = 0; s <= FDN; ++s) w[FDN + s] = feed(s);   // planes 0..6
     fill_pw(0, SA);
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
 
+    const int lrow = (pyl + FDN) * PP + x0 + FDN;          // LDS slot of the column
+    const int hrow = (hy - y0 + FDN) * PP + hx + FDN;      // LDS slot of the halo point
     // step z: S holds plane z's pointwise inputs; N receives plane z + 1's.
     auto step = [&](int z, PW &S, PW &N) {
-        T qf[CPT];
-#pragma unroll
-        for (int c = 0; c < CPT; ++c) qf[c] = feed(z + FDN + 1, c);
+        const T qf = feed(z + FDN + 1);
         fill_pw(z + 1, N);
         // stage plane z
         T *pl = lds + (z & 1) * G.plane;
-#pragma unroll
-        for (int c = 0; c < CPT; ++c)
-            if (has[c]) pl[(py + FDN) * PP + x0 + c + FDN] = w[c][FDN];
+        if (act) pl[lrow] = w[FDN];
+        if (hact) pl[hrow] = MODE == MODE_LANCZOS ? (S.hx - ca * S.hyv) * ib : S.hx;
         lds_barrier();
         if (act) {
-            const T *row = pl + (py + FDN) * PP + x0 + FDN;
+            const T *row = pl + lrow;
+            const R cv = (R)w[FDN];
+            R t = cv * c0d;
 #pragma unroll
-            for (int c = 0; c < CPT; ++c) {
-                if (!has[c]) continue;
-                const R cv = (R)w[c][FDN];
-                R t = cv * c0d;
-#pragma unroll
-                for (int a = 1; a <= FDN; ++a) {
-                    const R xl = (R)row[c - a], xr = (R)row[c + a];
-                    const R ym = (R)row[c - a * PP], yp = (R)row[c + a * PP];
-                    t += (xl + xr + ym + yp + (R)w[c][FDN - a] + (R)w[c][FDN + a]) * cl[a];
-                    // keep the LDS reads of one offset in flight at a time (register pressure)
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                const R hv = hl * t + (R)S.ve[c] * cv + (R)S.nl[c];
-                const int ozp = z * M2 + col + c;
-                if (MODE == MODE_HSUB) {
-                    Yb[ozp] = (T)hv;
-                } else if (CH) {
-                    R o;
-                    if (A.first) o = (hv - chi * cv) * izee;
-                    else o = (hv - chi * cv) * two_izee - (R)S.pv[c];
-                    const T os = (T)o;   // the stored value is the one reduced and projected
-                    Yob[ozp] = os;
-                    acc[0] += (double)os * (double)os;
-                    acc[1] += (double)os * cv;
-                    if (MODE == MODE_CHEBD) {
-                        if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
-                            const bool ctr = z == nloc && py == nloc && x0 + c == nloc;
-                            Db[ozp] = (T)((R)os * dk + (ctr ? d0 : 0.0));
-                        } else {
-                            Db[ozp] = (T)((R)S.dv[c] + (R)os * dk);
-                        }
+            for (int a = 1; a <= FDN; ++a) {
+                const R xl = (R)row[-a], xr = (R)row[a];
+                const R ym = (R)row[-a * PP], yp = (R)row[a * PP];
+                t += (xl + xr + ym + yp + (R)w[FDN - a] + (R)w[FDN + a]) * cl[a];
+                // keep the LDS reads of one offset in flight at a time (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const R hv = hl * t + (R)S.ve * cv + (R)S.nl;
+            const int ozp = z * M2 + col;
+            if (MODE == MODE_HSUB) {
+                Yb[ozp] = (T)hv;
+            } else if (CH) {
+                R o;
+                if (A.first) o = (hv - chi * cv) * izee;
+                else o = (hv - chi * cv) * two_izee - (R)S.pv;
+                const T os = (T)o;   // the stored value is the one reduced and projected
+                Yob[ozp] = os;
+                acc[0] += (double)os * (double)os;
+                acc[1] += (double)os * cv;
+                if (MODE == MODE_CHEBD) {
+                    if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
+                        const bool ctr = z == nloc && py == nloc && x0 == nloc;
+                        Db[ozp] = (T)((R)os * dk + (ctr ? d0 : 0.0));
+                    } else {
+                        Db[ozp] = (T)((R)S.dv + (R)os * dk);
                     }
-                    if (z == nloc && py == nloc && x0 + c == nloc) sh_center = (double)os;
-                } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
-                    const T q = (T)(hv - bp * (R)S.pv[c]);
-                    Xw[ozp] = q;
-                    Yb[ozp] = (T)cv;
-                    acc[0] += cv * hv;
-                    acc[1] += (double)q * (double)q;
-                    acc[2] += (double)q * cv;
-                    acc[3] += cv * cv;
                 }
+                if (z == nloc && py == nloc && x0 == nloc) sh_center = (double)os;
+            } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
+                const T q = (T)(hv - bp * (R)S.pv);
+                A.XO[base + ozp] = q;
+                A.YO[base + ozp] = (T)cv;
+                acc[0] += cv * hv;
+                acc[1] += (double)q * (double)q;
+                acc[2] += (double)q * cv;
+                acc[3] += cv * cv;
             }
         }
         // shift the window and insert plane z + 7
 #pragma unroll
-        for (int c = 0; c < CPT; ++c) {
-#pragma unroll
-            for (int s = 0; s < WIN - 1; ++s) w[c][s] = w[c][s + 1];
-            w[c][WIN - 1] = qf[c];
-        }
+        for (int s = 0; s < WIN - 1; ++s) w[s] = w[s + 1];
+        w[WIN - 1] = qf;
     };
     for (int z = 0; z < M; z += 2) {
         step(z, SA, SB);
@@ -425,15 +428,29 @@ __global__ void __launch_bounds__(LB) k_zmarch(ZArgs<T> A)
     if (MODE == MODE_HSUB) return;
     block_sum<4>(acc, red);
     if (tid == 0) {
+        const bool has_ctr = nloc >= y0 && nloc < y0 + ny;
         if (CH) {
-            A.center[b] = sh_center;
-            A.dots[2 * b] = acc[0];
-            A.dots[2 * b + 1] = acc[1];
+            if (has_ctr) A.center[b] = sh_center;
+            A.dots[(2 * (size_t)b + h) * 2] = acc[0];
+            A.dots[(2 * (size_t)b + h) * 2 + 1] = acc[1];
         } else {
-            double *o = A.lred + 4 * (size_t)b;
+            double *o = A.lred + (2 * (size_t)b + h) * 4;
             o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
         }
     }
+}
+
+// Up to 640 threads (M <= 35): at most 96 VGPRs, so that two workgroups
+// (20 waves) share a CU; larger cubes take up to 1024 threads, one per CU.
+template <typename T, int MODE>
+__global__ __attribute__((amdgpu_waves_per_eu(5, 8))) void __launch_bounds__(640) k_zmarch(ZArgs<T> A)
+{
+    zmarch_body<T, MODE>(A);
+}
+template <typename T, int MODE>
+__global__ void __launch_bounds__(1024) k_zmarch_big(ZArgs<T> A)
+{
+    zmarch_body<T, MODE>(A);
 }
 
 // ---------------------------------------------------------------- projections
@@ -738,12 +755,16 @@ __device__ __forceinline__ int first_cz_at_least(const ImgMeta *im, int cnt, int
     return lo;
 }
 
-template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU>
+template <typename T, int MAXP, int FORMED, int SZ, int NT, int NPU, int NM>
 __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
 {
-    constexpr int NV = NG * MAXP;
-    constexpr int LV = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : 0;
-    static_assert(LV > 0, "k_projs: NG * MAXP must be 8, 16 or 32");
+    // NM members per workgroup: all NG slots of the group, or a 2x2 block of
+    // its 4x2 slots (two workgroups per group, half the LDS each)
+    static_assert(NM == NG || NM == 4, "k_projs: NM is NG or 4");
+    constexpr int SUBS = NG / NM;
+    constexpr int NV = NM * MAXP;
+    constexpr int LV = NV == 32 ? 5 : NV == 16 ? 4 : NV == 8 ? 3 : NV == 4 ? 2 : 0;
+    static_assert(LV > 0, "k_projs: NM * MAXP must be 4, 8, 16 or 32");
     // projector values are loaded and summed for the first NPU of the MAXP
     // slots (3 for a table of p projectors); the other sums stay exact zeros
     static_assert(NPU >= 1 && NPU <= MAXP, "k_projs: NPU of MAXP slots");
@@ -751,34 +772,35 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
     constexpr int RPI = SZ >= 4 ? 2 : 3;   // rounds of 64 box points per U load batch
     constexpr int PPT = SZ >= 4 ? 2 : 3;   // staged points per thread per load batch
     const Geo &G = A.g;
-    const int q = blockIdx.x;
+    const int q = blockIdx.x / SUBS, hsub = blockIdx.x % SUBS;
     const int b0 = q * NG, gn = min(NG, A.B - b0);
     const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, M3 = G.M3;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwave = blockDim.x >> 6;
-    int ni[NG], nj[NG], nk[NG];
-    bool on[NG];
-    T cab[NG], ibb[NG];
+    int ni[NM], nj[NM], nk[NM], gs[NM];   // gs: the member's slot in the group
+    bool on[NM];
+    T cab[NM], ibb[NM];
     int nact = 0;
     unsigned onmask = 0;
     int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
 #pragma unroll
-    for (int m = 0; m < NG; ++m) {
-        on[m] = m < gn && !(G.done && G.done[b0 + m]);
-        const int node = G.node_list[b0 + min(m, gn - 1)];
+    for (int m = 0; m < NM; ++m) {
+        gs[m] = NM == NG ? m : (m & 1) + 2 * hsub + 4 * (m >> 1);
+        on[m] = gs[m] < gn && !(G.done && G.done[b0 + gs[m]]);
+        const int node = G.node_list[b0 + min(gs[m], gn - 1)];
         ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
         cab[m] = 0; ibb[m] = 1;
         if (on[m]) {
             ++nact;
             onmask |= 1u << m;
-            if (FORMED) { cab[m] = (T)A.ca[b0 + m]; ibb[m] = (T)A.ib[b0 + m]; }
+            if (FORMED) { cab[m] = (T)A.ca[b0 + gs[m]]; ibb[m] = (T)A.ib[b0 + gs[m]]; }
             x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
             y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
         }
     }
     const int cnt = A.glist_cnt[q];
     if (nact == 0 || cnt == 0) return;
-    // LDS: image data [cnt] | staged slab [SZ][pu][NG]
+    // LDS: image data [cnt] | staged slab [SZ][pu][NM]
     __shared__ int sh_ncmax, sh_lo, sh_hi;
     if (tid == 0) sh_ncmax = 0;
     __syncthreads();
@@ -796,21 +818,21 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
         atomicMax(&sh_ncmax, nc);
         // this group's projections start at zero; each slab adds its totals
 #pragma unroll
-        for (int m = 0; m < NG; ++m)
+        for (int m = 0; m < NM; ++m)
             if ((onmask >> m) & 1u)
-                for (int i = 0; i < e.np; ++i) A.s[(size_t)(b0 + m) * A.n_ip + e.ipb + i] = 0.0;
+                for (int i = 0; i < e.np; ++i) A.s[(size_t)(b0 + gs[m]) * A.n_ip + e.ipb + i] = 0.0;
     }
     const size_t gb = (size_t)gn * M3 * TS;
     const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
     const __amdgpu_buffer_rsrc_t rY = brsrc((FORMED ? A.V2 : A.V) + (size_t)b0 * M3, gb);
     const bool together = (x1 - x0 + M) * (y1 - y0 + M) <= A.pu;
 #pragma unroll 1
-    for (int sub = 0; sub < (together ? 1 : NG); ++sub) {
+    for (int sub = 0; sub < (together ? 1 : NM); ++sub) {
         // members staged in this walk, and the union of their cubes
-        bool in_sub[NG];
+        bool in_sub[NM];
         int ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30), uz0 = 1 << 30, uz1 = -(1 << 30);
 #pragma unroll
-        for (int m = 0; m < NG; ++m) {
+        for (int m = 0; m < NM; ++m) {
             in_sub[m] = on[m] && (together || m == sub);
             if (in_sub[m]) {
                 ux0 = min(ux0, ni[m] - nloc); ux1 = max(ux1, ni[m] + nloc);
@@ -839,7 +861,7 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
             // thread's points issued before the first store
 #pragma unroll 1
             for (int p0 = 0; p0 < sp; p0 += PPT * (int)blockDim.x) {
-                T xv[PPT][NG], yv[PPT][NG];
+                T xv[PPT][NM], yv[PPT][NM];
 #pragma unroll
                 for (int r = 0; r < PPT; ++r) {
                     const int p = p0 + r * (int)blockDim.x + tid;
@@ -847,11 +869,11 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                     const int yy = pq / ux, xx = pq - yy * ux;
                     const int gx = ux0 + xx, gy = uy0 + yy, gz = gzs + sl;
 #pragma unroll
-                    for (int m = 0; m < NG; ++m) {
+                    for (int m = 0; m < NM; ++m) {
                         const int lx = gx - ni[m] + nloc, ly = gy - nj[m] + nloc, lz = gz - nk[m] + nloc;
                         const bool ok = p < sp && in_sub[m] && (unsigned)lx < (unsigned)M &&
                                         (unsigned)ly < (unsigned)M && (unsigned)lz < (unsigned)M;
-                        const unsigned o = ok ? (unsigned)(m * M3 + lz * M2 + ly * M + lx) * TS : OOB;
+                        const unsigned o = ok ? (unsigned)(gs[m] * M3 + lz * M2 + ly * M + lx) * TS : OOB;
                         xv[r][m] = bload<T>(rX, o);
                         yv[r][m] = FORMED ? bload<T>(rY, o) : (T)0;
                     }
@@ -861,8 +883,8 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                     const int p = p0 + r * (int)blockDim.x + tid;
                     if (p < sp) {
 #pragma unroll
-                        for (int m = 0; m < NG; ++m)
-                            pl[(size_t)p * NG + m] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
+                        for (int m = 0; m < NM; ++m)
+                            pl[(size_t)p * NM + m] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
                     }
                 }
             }
@@ -919,13 +941,13 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                     for (int r = 0; r < RPI; ++r) {
 #pragma unroll
                         for (int z = 0; z < SZ; ++z) {
-                            const T *pv = pl + ((size_t)z * up + pp[r]) * NG;
-                            T v[NG];
+                            const T *pv = pl + ((size_t)z * up + pp[r]) * NM;
+                            T v[NM];
 #pragma unroll
-                            for (int m = 0; m < NG; ++m) v[m] = pv[m];
+                            for (int m = 0; m < NM; ++m) v[m] = pv[m];
                             if (in[r]) {
 #pragma unroll
-                                for (int m = 0; m < NG; ++m)
+                                for (int m = 0; m < NM; ++m)
 #pragma unroll
                                     for (int i = 0; i < NPU; ++i)
                                         acc[m * MAXP + i] += (double)v[m] * (double)u[r][z][i];
@@ -936,7 +958,245 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
                 const double tot = wave_sums<NV>(acc, lane);
                 // one lane per value and slab; slabs are ordered by the fence
                 // and barrier above, so the additions happen in slab order
-                if (writer) unsafeAtomicAdd(A.s + (size_t)(b0 + mw) * A.n_ip + ipb + iw, tot * idn);
+                const int gw = NM == NG ? mw : (mw & 1) + 2 * hsub + 4 * (mw >> 1);   // gs[mw]
+                if (writer) unsafeAtomicAdd(A.s + (size_t)(b0 + gw) * A.n_ip + ipb + iw, tot * idn);
+            }
+        }
+    }
+}
+
+// The projections of k_projs with every image owned by one wave for its
+// whole z-extent.  The workgroup walks the union of its members' cubes in
+// slabs of SZ planes staged in LDS as in k_projs; image li of the z-sorted
+// group list belongs to wave li mod nwave and to that wave's register slot
+// (li / nwave) mod R from the slab where its box enters the union to the
+// slab where it leaves.  Lane l keeps, for each slot, the partial sums of the
+// box points whose xy index is l mod 64 over every plane, so the wave reduces
+// (wave_sums) and stores each projection once, when its image leaves: no
+// per-slab reduction, no atomics, no zeroing pass.  Per lane and member the
+// products are added plane by plane, z ascending, in the same round order
+// for every grouping of the nodes, so each member's totals depend only on its
+// own cube.  An image whose box misses the union in z contributes zero.
+// Requires one box size for every image of the list and at most R images of
+// a wave alive at once (k_group_images checks both per group).
+template <typename T, int FORMED, int SZ, int NT, int NM, int R, int RPI>
+__global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
+{
+    static_assert(NM == NG || NM == 4, "k_projr: NM is NG or 4");
+    constexpr int SUBS = NG / NM;
+    constexpr int NV = NM;                    // one projector per image (s channel)
+    constexpr int LV = NV == 8 ? 3 : 2;
+    constexpr unsigned TS = sizeof(T);
+    constexpr int PPT = 2;                    // staged points per thread per load batch
+    const Geo &G = A.g;
+    const int q = blockIdx.x / SUBS, hsub = blockIdx.x % SUBS;
+    const int b0 = q * NG, gn = min(NG, A.B - b0);
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, M3 = G.M3;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int NW = NT / 64;
+    int ni[NM], nj[NM], nk[NM], gs[NM];
+    bool on[NM];
+    T cab[NM], ibb[NM];
+    int nact = 0;
+    unsigned onmask = 0;
+    int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        gs[m] = NM == NG ? m : (m & 1) + 2 * hsub + 4 * (m >> 1);
+        on[m] = gs[m] < gn && !(G.done && G.done[b0 + gs[m]]);
+        const int node = G.node_list[b0 + min(gs[m], gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+        cab[m] = 0; ibb[m] = 1;
+        if (on[m]) {
+            ++nact;
+            onmask |= 1u << m;
+            if (FORMED) { cab[m] = (T)A.ca[b0 + gs[m]]; ibb[m] = (T)A.ib[b0 + gs[m]]; }
+            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
+            y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
+        }
+    }
+    const int cnt = A.glist_cnt[q];
+    if (nact == 0 || cnt == 0) return;
+    // LDS: image data [cnt] | staged slab [SZ][NM][padded plane]
+    extern __shared__ unsigned char smem_raw[];
+    ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw);
+    T *pl = reinterpret_cast<T *>(smem_raw + ((size_t)cnt * sizeof(ImgMeta) + 15) / 16 * 16);
+    for (int li = tid; li < cnt; li += blockDim.x) {
+        const int J = A.glist[(size_t)q * GLCAP + li];
+        const int t = A.img_type[J], nc = A.nlocc[t];
+        ImgMeta e;
+        e.ox = A.img_nn[3 * J] - nc; e.oy = A.img_nn[3 * J + 1] - nc; e.oz = A.img_nn[3 * J + 2] - nc;
+        e.w = 2 * nc + 1; e.np = A.nproj[t]; e.t = t; e.ipb = A.ip_base[J];
+        e.uoff = A.udv_off[J];
+        im[li] = e;
+    }
+    __syncthreads();
+    const size_t gb = (size_t)gn * M3 * TS;
+    const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
+    const __amdgpu_buffer_rsrc_t rY = brsrc((FORMED ? A.V2 : A.V) + (size_t)b0 * M3, gb);
+    // LDS rows of the slab are padded to a pitch congruent to the box side
+    // modulo 32 values, so that the box rows one wave reads fall on disjoint
+    // banks; A.pu is the slab capacity per plane and member
+    const int wbox = __builtin_amdgcn_readfirstlane(im[0].w);
+    auto upitch_of = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
+    const bool together = upitch_of(x1 - x0 + M) * (y1 - y0 + M) <= A.pu;
+    // the value this lane stores for a leaving image: member kw
+    const int kw = lane >> (6 - LV);
+    const bool wlane = (lane & ((64 >> LV) - 1)) == 0;
+    const int gw = NM == NG ? kw : (kw & 1) + 2 * hsub + 4 * (kw >> 1);   // gs[kw]
+    double *sw = A.s + (size_t)(b0 + gw) * A.n_ip;
+#pragma unroll 1
+    for (int sub = 0; sub < (together ? 1 : NM); ++sub) {
+        bool in_sub[NM];
+        unsigned submask = 0;
+        int ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30), uz0 = 1 << 30, uz1 = -(1 << 30);
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            in_sub[m] = on[m] && (together || m == sub);
+            if (in_sub[m]) {
+                submask |= 1u << m;
+                ux0 = min(ux0, ni[m] - nloc); ux1 = max(ux1, ni[m] + nloc);
+                uy0 = min(uy0, nj[m] - nloc); uy1 = max(uy1, nj[m] + nloc);
+                uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc);
+            }
+        }
+        if (ux1 < ux0) continue;
+        const bool writer = wlane && ((submask >> kw) & 1u);
+        const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, upx = upitch_of(ux), up = upx * uy;
+        const int np_ = ux * uy, sp = SZ * np_;   // staged points of a slab (unpadded)
+        const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
+        // per-wave slot state: started images, live-slot mask, per-slot image data
+        int nstart = 0;
+        unsigned live = 0;
+        int s_oz[R], s_ze[R], s_w[R], s_ipb[R], s_t[R];
+        long s_uoff[R];
+        int s_pp[R][RPI];                     // LDS point of the lane's box point, or -1
+        double acc[R][NM];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            s_oz[r] = s_ze[r] = s_w[r] = s_ipb[r] = s_t[r] = 0;
+            s_uoff[r] = 0;
+#pragma unroll
+            for (int k = 0; k < RPI; ++k) s_pp[r][k] = -1;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) acc[r][m] = 0.0;
+        }
+#pragma unroll 1
+        for (int gzs = zs0; gzs <= uz1; gzs += SZ) {
+            // the previous slab is consumed before it is overwritten
+            __syncthreads();
+#pragma unroll 1
+            for (int p0 = 0; p0 < sp; p0 += PPT * (int)blockDim.x) {
+                T xv[PPT][NM], yv[PPT][NM];
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    const int sl = p / np_, pq = p - sl * np_;
+                    const int yy = pq / ux, xx = pq - yy * ux;
+                    const int gx = ux0 + xx, gy = uy0 + yy, gz = gzs + sl;
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) {
+                        const int lx = gx - ni[m] + nloc, ly = gy - nj[m] + nloc, lz = gz - nk[m] + nloc;
+                        const bool ok = p < sp && in_sub[m] && (unsigned)lx < (unsigned)M &&
+                                        (unsigned)ly < (unsigned)M && (unsigned)lz < (unsigned)M;
+                        const unsigned o = ok ? (unsigned)(gs[m] * M3 + lz * M2 + ly * M + lx) * TS : OOB;
+                        xv[r][m] = bload<T>(rX, o);
+                        yv[r][m] = FORMED ? bload<T>(rY, o) : (T)0;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    if (p < sp) {
+                        // member-major: [plane][member][row][padded column]
+                        const int sl = p / np_, pq = p - sl * np_;
+                        const int yy = pq / ux, xx = pq - yy * ux;
+                        T *dst = pl + (size_t)sl * NM * up + yy * upx + xx;
+#pragma unroll
+                        for (int m = 0; m < NM; ++m)
+                            dst[(size_t)m * up] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
+                    }
+                }
+            }
+            __syncthreads();
+            const int zl = gzs + SZ - 1;   // last plane of the slab
+            // images of this wave that enter the union in this slab take their slot
+#pragma unroll 1
+            for (;;) {
+                const int li = wave + NW * nstart;
+                if (li >= cnt) break;
+                const int oz = __builtin_amdgcn_readfirstlane(im[li].oz);
+                const int w = __builtin_amdgcn_readfirstlane(im[li].w);
+                const bool miss = oz + w - 1 < uz0 || oz > uz1;
+                const int zs = max(oz, uz0);
+                if (!miss && zs > zl) break;
+                if (miss) {   // no plane of the box in the union: the projection is zero
+                    const int ipb = __builtin_amdgcn_readfirstlane(im[li].ipb);
+                    if (writer) sw[ipb] = 0.0;
+                    ++nstart;
+                    continue;
+                }
+                const int slot = nstart % R;
+                const int ox = __builtin_amdgcn_readfirstlane(im[li].ox), oy = __builtin_amdgcn_readfirstlane(im[li].oy);
+                const int rlo = max(0, uy0 - oy), rhi = min(w - 1, uy1 - oy);
+                const int clo = max(0, ux0 - ox), chi = min(w - 1, ux1 - ox);
+                const float rw = 1.0f / (float)w;
+                int pp[RPI];
+#pragma unroll
+                for (int k = 0; k < RPI; ++k) {
+                    const int a = 64 * k + lane;
+                    const int by = (int)(((float)a + 0.5f) * rw), bx = a - by * w;
+                    const bool in = a < w * w && by >= rlo && by <= rhi && bx >= clo && bx <= chi;
+                    pp[k] = in ? (oy + by - uy0) * upx + (ox + bx - ux0) : -1;
+                }
+                const int ipb = __builtin_amdgcn_readfirstlane(im[li].ipb), t = __builtin_amdgcn_readfirstlane(im[li].t);
+                const long uoff = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(im[li].uoff >> 32)) << 32) |
+                                         (unsigned)__builtin_amdgcn_readfirstlane((int)im[li].uoff));
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r == slot) {
+                        s_oz[r] = oz; s_ze[r] = min(oz + w - 1, uz1); s_w[r] = w; s_ipb[r] = ipb; s_t[r] = t;
+                        s_uoff[r] = uoff;
+#pragma unroll
+                        for (int k = 0; k < RPI; ++k) s_pp[r][k] = pp[k];
+#pragma unroll
+                        for (int m = 0; m < NM; ++m) acc[r][m] = 0.0;
+                    }
+                }
+                live |= 1u << slot;
+                ++nstart;
+            }
+            // every live slot adds the slab's planes of its box
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!((live >> r) & 1u)) continue;
+                const int w = s_w[r], w2 = w * w;
+                const int za = max(gzs, max(s_oz[r], uz0)), zb = min(zl, s_ze[r]);
+                const __amdgpu_buffer_rsrc_t rU = brsrc(A.udv + s_uoff[r], (size_t)w2 * w * TS);
+                for (int z = za; z <= zb; ++z) {
+                    const int bz = z - s_oz[r];
+                    T u[RPI];
+#pragma unroll
+                    for (int k = 0; k < RPI; ++k)
+                        u[k] = bload<T>(rU, s_pp[r][k] >= 0 ? (unsigned)(bz * w2 + 64 * k + lane) * TS : OOB);
+#pragma unroll
+                    for (int k = 0; k < RPI; ++k) {
+                        const T *pv = pl + (size_t)(z - gzs) * NM * up + max(s_pp[r][k], 0);
+                        T v[NM];
+#pragma unroll
+                        for (int m = 0; m < NM; ++m) v[m] = pv[(size_t)m * up];
+                        if (s_pp[r][k] >= 0) {
+#pragma unroll
+                            for (int m = 0; m < NM; ++m) acc[r][m] += (double)v[m] * (double)u[k];
+                        }
+                    }
+                }
+                if (s_ze[r] <= zl) {   // the image leaves the union in this slab
+                    const double tot = wave_sums<NV>(acc[r], lane);
+                    if (writer) sw[s_ipb[r]] = tot * A.inv_denom[16 * s_t[r]];
+                    live &= ~(1u << r);
+                }
             }
         }
     }
@@ -1195,6 +1455,11 @@ struct LArgs {
     const int *img_type, *img_nn, *nlocc, *nproj;
     int *list, *list_cnt, *overflow;
     int *glist, *glist_cnt;       // [ceil(B/NG)][GLCAP], [ceil(B/NG)]
+    // k_projr eligibility of every group (set to 1 if any fails): one box
+    // size in the list, and at most pr_R live images per wave of pr_nw waves
+    // walking slabs of pr_sz planes
+    int *pr_bad;
+    int pr_R, pr_nw, pr_sz;
 };
 
 __global__ void k_node_images(LArgs A, int B)
@@ -1264,6 +1529,31 @@ __global__ void k_group_images(LArgs A, int B)
         L[j + 1] = J;
     }
     A.glist_cnt[q] = cnt;
+    if (A.pr_bad && cnt > 0) {
+        // the slot plan of k_projr over the union of all the group's cubes
+        // (any sub-walk clamps to a smaller z-range and only shortens lives)
+        bool bad = false;
+        const int nc0 = A.nlocc[A.img_type[L[0]]];
+        for (int i = 1; i < cnt; ++i) bad |= A.nlocc[A.img_type[L[i]]] != nc0;
+        const int w = 2 * nc0 + 1, SZ = A.pr_sz, RR = min(A.pr_R, 16);
+        int uz0 = 1 << 30, uz1 = -(1 << 30);
+        for (int m = 0; m < gn; ++m) { uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc); }
+        const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
+        for (int wv = 0; wv < A.pr_nw && !bad; ++wv) {
+            int ring[16];
+            for (int r = 0; r < 16; ++r) ring[r] = -(1 << 30);
+            int k = 0;
+            for (int li = wv; li < cnt && !bad; li += A.pr_nw, ++k) {
+                const int oz = A.img_nn[3 * L[li] + 2] - nc0;
+                if (oz + w - 1 < uz0 || oz > uz1) continue;   // no slot
+                const int S = (max(oz, uz0) - zs0) / SZ, E = (min(oz + w - 1, uz1) - zs0) / SZ;
+                const int slot = k % RR;
+                if (ring[slot] >= S) bad = true;
+                ring[slot] = E;
+            }
+        }
+        if (bad) atomicOr(A.pr_bad, 1);
+    }
 }
 
 // ---------------------------------------------------------------- fills
@@ -1289,7 +1579,7 @@ __global__ void k_fill_delta(T *p, int M3, int B, int c)
 struct QArgs {
     int B, pass;
     double tol;
-    const double *lred;       // [B][4]
+    const double *lred;       // [B][2 halves][4]
     double *aa, *bb;          // [MAXIT+2][B] (pass-major: the lanes of a wave read one row)
     double *d, *e;            // scratch [B][MAXIT+2]
     double *lmin, *lmax, *lmin_prev, *lmax_prev;
@@ -1350,36 +1640,70 @@ __device__ int ql_extremes(double *d, double *e, int n, double *lmin, double *lm
     return 0;
 }
 
-// Largest (sign > 0) or smallest (sign < 0) eigenvalue of the symmetric
-// tridiagonal (d[0..n-1], e[0..n-2]) by Newton's method on det(xI - T) from a
-// start x0 outside the spectrum.  With q_i = p_i/p_{i-1} (Sturm ratios) and
-// r_i = q_i'/q_i, p'/p = sum_i q_i'/q_i; past the extreme root the
+// Largest and smallest eigenvalue of the symmetric tridiagonal (d[0..n-1],
+// e[0..n-2], strided by st elements) by Newton's method on det(xI - T), from
+// starts xh above and xl below the spectrum.  With q_i = p_i/p_{i-1} (Sturm
+// ratios) and r_i = q_i'/q_i, p'/p = sum_i q_i'/q_i; past an extreme root the
 // characteristic polynomial is monotone and convex (concave on the left), so
-// the iterates move monotonically onto the root and the loop stops at the
-// first step that no longer moves inward.  Returns false after maxit steps.
-// d and e are strided by st elements.
-__device__ bool tridiag_extreme(const double *d, const double *e, size_t st, int n, double x0, int sign,
-                                double *root)
+// the iterates move monotonically onto the root and each one stops at the
+// first step that no longer moves inward.  Both iterations share every sweep
+// over the rows, which are read in chunks of CH with the next chunk's loads
+// in flight (the rows live in L2, a few hundred cycles away).  Returns false
+// when either needs more than 200 steps.
+__device__ bool tridiag_extremes(const double *d, const double *e, size_t st, int n, double xh, double xl,
+                                 double *hi, double *lo)
 {
-    double x = x0;
+#pragma clang fp contract(off)
+    constexpr int CH = 8;
+    bool dh = false, dl = false;
     for (int it = 0; it < 200; ++it) {
-        double q = x - d[0], dq = 1.0;
-        double iq = 1.0 / q;
-        double s = dq * iq;
-        for (int i = 1; i < n; ++i) {
-            const double em = e[(size_t)(i - 1) * st];
-            const double r = em * em * iq;
-            dq = 1.0 + r * dq * iq;
-            q = (x - d[(size_t)i * st]) - r;
-            iq = 1.0 / q;
-            s += dq * iq;
+        double qh = xh - d[0], ql = xl - d[0], dqh = 1.0, dql = 1.0;
+        double iqh = 1.0 / qh, iql = 1.0 / ql;
+        double sh = dqh * iqh, sl = dql * iql;
+        double dn[CH], en[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = 1 + j;
+            dn[j] = i < n ? d[(size_t)i * st] : 0.0;
+            en[j] = i < n ? e[(size_t)(i - 1) * st] : 0.0;
         }
-        const double xn = x - 1.0 / s;
-        if (!(sign > 0 ? xn < x : xn > x)) {
-            *root = x;
-            return true;
+        for (int i0 = 1; i0 < n; i0 += CH) {
+            double dc[CH], ec[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) { dc[j] = dn[j]; ec[j] = en[j]; }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int i = i0 + CH + j;
+                dn[j] = i < n ? d[(size_t)i * st] : 0.0;
+                en[j] = i < n ? e[(size_t)(i - 1) * st] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                if (i0 + j < n) {
+                    const double em2 = ec[j] * ec[j];
+                    const double rh = em2 * iqh, rl = em2 * iql;
+                    dqh = 1.0 + rh * dqh * iqh;
+                    dql = 1.0 + rl * dql * iql;
+                    qh = (xh - dc[j]) - rh;
+                    ql = (xl - dc[j]) - rl;
+                    iqh = 1.0 / qh;
+                    iql = 1.0 / ql;
+                    sh += dqh * iqh;
+                    sl += dql * iql;
+                }
+            }
         }
-        x = xn;
+        if (!dh) {
+            const double xn = xh - 1.0 / sh;
+            if (!(xn < xh)) { *hi = xh; dh = true; }
+            else xh = xn;
+        }
+        if (!dl) {
+            const double xn = xl - 1.0 / sl;
+            if (!(xn > xl)) { *lo = xl; dl = true; }
+            else xl = xn;
+        }
+        if (dh && dl) return true;
     }
     return false;
 }
@@ -1390,7 +1714,9 @@ __global__ void k_lanczos_scalars(QArgs A)
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= A.B || A.done[b]) return;
     const int S = MAXIT + 2;
-    const double *r = A.lred + 4 * (size_t)b;
+    // the two row halves of the z-march, added in half order
+    const double *r0 = A.lred + 8 * (size_t)b, *r1 = r0 + 4;
+    const double r[4] = {r0[0] + r1[0], r0[1] + r1[1], r0[2] + r1[2], r0[3] + r1[3]};
     const int p = A.pass;
     const double al = r[0];
     double b2 = r[1] - 2.0 * al * r[2] + al * al * r[3];
@@ -1422,7 +1748,7 @@ __global__ void k_lanczos_scalars(QArgs A)
     }
     up += 8.0 * 2.2e-16 * fabs(up) + 1e-300;
     dn -= 8.0 * 2.2e-16 * fabs(dn) + 1e-300;
-    if (!tridiag_extreme(ad, be2, st, nq, up, 1, &hi) || !tridiag_extreme(ad, be2, st, nq, dn, -1, &lo)) {
+    if (!tridiag_extremes(ad, be2, st, nq, up, dn, &hi, &lo)) {
         double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
         for (int i = 0; i < nq; ++i) { d[i] = ad[(size_t)i * st]; e[i] = be2[(size_t)i * st]; }
         if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
@@ -1468,8 +1794,9 @@ __global__ void k_moments(MArgs A)
     }
     m[k] = (double)(A.center[b] / A.h3);
     if (4 * k > A.npl) {
-        m[2 * k] = (double)((2 * A.dots[2 * b] - 1) / A.h3);
-        m[2 * k - 1] = (double)((2 * A.dots[2 * b + 1] - A.htemp[b]) / A.h3);
+        const double *d0 = A.dots + 4 * (size_t)b, *d1 = d0 + 2;   // the two row halves
+        m[2 * k] = (double)((2 * (d0[0] + d1[0]) - 1) / A.h3);
+        m[2 * k - 1] = (double)((2 * (d0[1] + d1[1]) - A.htemp[b]) / A.h3);
     }
 }
 

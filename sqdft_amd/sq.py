@@ -27,6 +27,21 @@ from .system import System
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
+def node_order(n: int) -> np.ndarray:
+    """The order in which sqg_density visits the n^3 FD nodes and cuts them
+    into batches: 4x4x2 bricks of nodes (x fastest inside a brick, bricks x,
+    then y, then z), so that consecutive batch slots hold overlapping cubes."""
+    out = []
+    for z0 in range(0, n, 2):
+        for y0 in range(0, n, 4):
+            for x0 in range(0, n, 4):
+                for z in range(z0, min(z0 + 2, n)):
+                    for y in range(y0, min(y0 + 4, n)):
+                        for x in range(x0, min(x0 + 4, n)):
+                            out.append((z * n + y) * n + x)
+    return np.asarray(out, dtype=np.int32)
+
+
 class SpectralQuadrature:
     def __init__(self, system: System, precision: str = "fp64", batch_nodes: int = 0,
                  device: int = 0, ckpt_interval: int = 8, fermi_tol: float | None = None,

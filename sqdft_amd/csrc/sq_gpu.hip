@@ -77,8 +77,8 @@ struct sqg_ctx {
     DBuf X, Y, X2, Y2, NLv, s, node_list, list, list_cnt, flags;   // X2, Y2: Lanczos ping-pong
     DBuf glist, glist_cnt;         // group image lists (NG slots per group)
     int gl_max = 0;                // longest group list of the current batch
-    bool projr_ok = false;         // k_projr's slot plan holds for every group of the batch
-    DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, cnt, done, ca, ib, bprev;
+    int projr_R = 0;               // k_projr register slots for the batch (0: k_projs instead)
+    DBuf lred, aa, bb, dq, eq, lmin, lmax, lmin_prev, lmax_prev, lchg, cnt, done, ca, ib, bprev;
     DBuf chiB, zeeB, center, dots, htemp;
     // all-node state of the last density step
     DBuf veff, chi_all, zee_all, lmin_all, lmax_all, iters_all, mu_all, W, Ci_all, rho_d, part;
@@ -180,6 +180,7 @@ int ensure_batch(sqg_ctx *c, int B)
                     &c->chiB, &c->zeeB, &c->center, &c->htemp})
         HIPCHK(d->ensure((size_t)B * sizeof(double)));
     HIPCHK(c->dots.ensure((size_t)B * ZHALF * 2 * sizeof(double)));
+    HIPCHK(c->lchg.ensure((size_t)B * 2 * sizeof(double)));
     HIPCHK(c->cnt.ensure((size_t)B * sizeof(int)));
     HIPCHK(c->done.ensure((size_t)B * sizeof(int)));
     c->Bcap = B;
@@ -254,6 +255,7 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
 // Shape of k_projr (s-channel tables only): members per workgroup, register
 // slots per wave and planes per slab; nm = 0 when it does not apply.
 constexpr int PROJR_NT = 512, PROJR_RPI = 3;
+constexpr int PROJR_SLOTS[3] = {5, 10, 16};   // compiled slot counts (five keep FP64 out of scratch)
 struct ProjrShape { int nm, R, sz; };
 ProjrShape projr_shape(const sqg_ctx *c)
 {
@@ -262,16 +264,25 @@ ProjrShape projr_shape(const sqg_ctx *c)
     for (int t = 0; t < c->n_typ; ++t)
         if (c->h_nproj[t] > 0) w = std::max(w, 2 * c->h_nlocc[t] + 1);
     if (w * w > 64 * PROJR_RPI) return {0, 0, 0};
-    ProjrShape s{8, 6, 1};
-    if (const char *e = std::getenv("SQG_PROJR")) {   // experiment: "<nm>:<R>:<sz>" or "off"
-        if (std::strcmp(e, "off") == 0) return {0, 0, 0};
-        std::sscanf(e, "%d:%d:%d", &s.nm, &s.R, &s.sz);
-    }
-    return s;
+    // eight members per workgroup; FP32 slabs of two planes fit the LDS
+    return {8, 0, c->p.precision == SQG_MIXED ? 2 : 1};
+}
+
+// k_projr's slab capacity per plane and member: the union plane of a 4x2
+// (NM = 8) or 2x2 (NM = 4) block of cubes, or one cube, with rows padded to a
+// pitch congruent to the box side modulo 32 (as k_projr pads them).
+int projr_pu(const sqg_ctx *c, const ProjrShape &ps)
+{
+    int wbox = 0;
+    for (int t = 0; t < c->n_typ; ++t)
+        if (c->h_nproj[t] > 0) wbox = 2 * c->h_nlocc[t] + 1;
+    auto upitch = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
+    const int ux = ps.nm == 8 ? c->M + 3 : c->M + 1, uy = c->M + 1;
+    return std::max(upitch(ux) * uy, upitch(c->M) * c->M);
 }
 
 // Image lists of the B nodes in node_list (per node and per group of NG
-// slots); sets gl_max and projr_ok.  Overflow goes to flags[0].
+// slots); sets gl_max and projr_R.  Overflow goes to flags[0].
 int node_images(sqg_ctx *c, int B)
 {
     LArgs L{};
@@ -290,7 +301,7 @@ int node_images(sqg_ctx *c, int B)
     HIPCHK(hipMemsetAsync(c->flags.as<int>() + 4, 0, sizeof(int), c->st));
     if (ps.nm) {
         L.pr_bad = c->flags.as<int>() + 4;
-        L.pr_R = ps.R; L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz;
+        L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz; L.pr_pu = projr_pu(c, ps);
     }
     hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
     const int ng = (B + NG - 1) / NG;
@@ -301,7 +312,12 @@ int node_images(sqg_ctx *c, int B)
     HIPCHK(hipStreamSynchronize(c->st));
     c->gl_max = 0;
     for (int i = 0; i < ng; ++i) c->gl_max = std::max(c->gl_max, gc[i]);
-    c->projr_ok = ps.nm != 0 && gc[ng] == 0;
+    // the smallest slot count that holds every group of the batch: k_projr
+    // gives the same sums with any slot count, so batches stay bitwise alike
+    c->projr_R = 0;
+    if (ps.nm)
+        for (int r : PROJR_SLOTS)
+            if (gc[ng] <= r) { c->projr_R = r; break; }
     return SQG_OK;
 }
 
@@ -350,9 +366,9 @@ void projs_np(sqg_ctx *c, dim3 grid, size_t shs, const PArgs<T> &P, int npu)
     else projs_go<T, 4, FORMED, SZ, NT, 4, NM>(c, grid, shs, P);
 }
 
-// Staged projections: members per workgroup (NM), planes per slab (sz) and
-// the largest union plane (pu) staged at once.  Returns false when not even
-// a single cube plane fits in LDS.
+// Staged projections (k_projs, all NG members of a group per workgroup):
+// planes per slab (sz) and the largest union plane (pu) staged at once.
+// Returns false when not even a single cube plane fits in LDS.
 template <typename T, int FORMED>
 bool launch_projs(sqg_ctx *c, dim3 grid, PArgs<T> &P, int mp)
 {
@@ -360,31 +376,15 @@ bool launch_projs(sqg_ctx *c, dim3 grid, PArgs<T> &P, int mp)
     const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
     // projector values are loaded and summed for npu of the mp slots
     const int npu = mp == 1 ? 1 : c->max_nproj == 3 ? 3 : 4;
-    int want_nm = 8, want_sz = 0;
-    if (const char *e = std::getenv("SQG_PROJ_VARIANT")) {   // experiment: "<nm>:<sz>"
-        want_nm = std::atoi(e);
-        const char *k = std::strchr(e, ':');
-        if (k) want_sz = std::atoi(k + 1);
-    }
-    for (int nm : {want_nm, 8}) {
-        const int pus[2] = {nm == 8 ? (c->M + 3) * (c->M + 3) : (c->M + 1) * (c->M + 1), c->M2};
-        for (int pu : pus) {
-            for (int sz : {D, D / 2, 1}) {
-                if (want_sz && sz != want_sz && nm == want_nm) continue;
-                const size_t shs = sa + (size_t)pu * nm * sizeof(T) * sz;
-                if (shs > 160 * 1024) continue;
-                P.pu = pu;
-                if (nm == 4) {
-                    if (sz == 4) projs_np<T, FORMED, 4, 4>(c, grid, shs, P, npu);
-                    else if (sz == 2) projs_np<T, FORMED, 2, 4>(c, grid, shs, P, npu);
-                    else projs_np<T, FORMED, 1, 4>(c, grid, shs, P, npu);
-                } else {
-                    if (sz == 4) projs_np<T, FORMED, 4, 8>(c, grid, shs, P, npu);
-                    else if (sz == 2) projs_np<T, FORMED, 2, 8>(c, grid, shs, P, npu);
-                    else projs_np<T, FORMED, 1, 8>(c, grid, shs, P, npu);
-                }
-                return true;
-            }
+    for (int pu : {(c->M + 3) * (c->M + 3), c->M2}) {
+        for (int sz : {D, D / 2}) {
+            const size_t shs = sa + (size_t)pu * NG * sizeof(T) * sz;
+            if (shs > 160 * 1024) continue;
+            P.pu = pu;
+            if (sz == 4) projs_np<T, FORMED, 4, NG>(c, grid, shs, P, npu);
+            else if (sz == 2) projs_np<T, FORMED, 2, NG>(c, grid, shs, P, npu);
+            else projs_np<T, FORMED, 1, NG>(c, grid, shs, P, npu);
+            return true;
         }
     }
     return false;
@@ -403,24 +403,15 @@ bool launch_projr(sqg_ctx *c, dim3 grid, PArgs<T> &P)
 {
     const ProjrShape ps = projr_shape(c);
     const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
-    // slab capacity per plane and member: the union plane of a 4x2 (NM = 8)
-    // or 2x2 (NM = 4) block of cubes, or one cube, with rows padded to a pitch
-    // congruent to the box side modulo 32 (as k_projr pads them)
-    int wbox = 0;
-    for (int t = 0; t < c->n_typ; ++t)
-        if (c->h_nproj[t] > 0) wbox = 2 * c->h_nlocc[t] + 1;
-    auto upitch = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
-    const int ux = ps.nm == 8 ? c->M + 3 : c->M + 1, uy = c->M + 1;
-    const int pu = std::max(upitch(ux) * uy, upitch(c->M) * c->M);
+    const int pu = projr_pu(c, ps);
     const size_t shm = sa + (size_t)pu * ps.nm * sizeof(T) * ps.sz;
     if (shm > 160 * 1024) return false;
     P.pu = pu;
-    if (ps.nm == 8 && ps.R == 5 && ps.sz == 1) projr_go<T, FORMED, 1, 8, 5>(c, grid, shm, P);
-    else if (ps.nm == 8 && ps.R == 6 && ps.sz == 1) projr_go<T, FORMED, 1, 8, 6>(c, grid, shm, P);
-    else if (ps.nm == 8 && ps.R == 6 && ps.sz == 2) projr_go<T, FORMED, 2, 8, 6>(c, grid, shm, P);
-    else if (ps.nm == 4 && ps.R == 8 && ps.sz == 1) projr_go<T, FORMED, 1, 4, 8>(c, grid, shm, P);
-    else if (ps.nm == 4 && ps.R == 8 && ps.sz == 2) projr_go<T, FORMED, 2, 4, 8>(c, grid, shm, P);
-    else return false;
+    constexpr int SZ = sizeof(T) == 8 ? 1 : 2;
+    if (ps.sz != SZ) return false;
+    if (c->projr_R == 5) projr_go<T, FORMED, SZ, 8, 5>(c, grid, shm, P);
+    else if (c->projr_R == 10) projr_go<T, FORMED, SZ, 8, 10>(c, grid, shm, P);
+    else projr_go<T, FORMED, SZ, 8, 16>(c, grid, shm, P);
     return true;
 }
 
@@ -439,7 +430,7 @@ void launch_proj(sqg_ctx *c, int B, const PArgs<T> &A, int which = -1, double al
         PArgs<T> P = A;
         P.B = B;
         const dim3 grid((B + NG - 1) / NG);
-        if (c->projr_ok && launch_projr<T, FORMED>(c, grid, P)) goto apply;
+        if (c->projr_R && launch_projr<T, FORMED>(c, grid, P)) goto apply;
         // staged projections (k_projs) with the deepest slab whose planes
         // (a union of a 4x4 footprint, else a single cube plane) fit in LDS
         const int mp = c->max_nproj <= 1 ? 1 : c->max_nproj <= 4 ? 4 : 0;
@@ -504,7 +495,7 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
     }
     // FP32 workgroups of up to 640 threads (M <= 35) are held to 96 VGPRs so
     // that two share a CU; FP64 ones would spill there and run one per CU
-    if (sizeof(T) == 4 && zmarch_threads(c->M) <= 640 && !std::getenv("SQG_ZBIG")) zmarch_k<T, MODE, 640>(c, B, A, shm);
+    if (sizeof(T) == 4 && zmarch_threads(c->M) <= 640) zmarch_k<T, MODE, 640>(c, B, A, shm);
     else zmarch_k<T, MODE, 1024>(c, B, A, shm);
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
@@ -532,6 +523,7 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     HIPCHK(hipMemsetAsync(c->cnt.p, 0, (size_t)B * sizeof(int), c->st));
     HIPCHK(hipMemsetAsync(c->lmin_prev.p, 0, (size_t)B * sizeof(double), c->st));
     HIPCHK(hipMemsetAsync(c->lmax_prev.p, 0, (size_t)B * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(c->lchg.p, 0, (size_t)B * 2 * sizeof(double), c->st));
     HIPCHK(hipMemsetAsync(c->ca.p, 0, (size_t)B * sizeof(double), c->st));
     HIPCHK(hipMemsetAsync(c->bprev.p, 0, (size_t)B * sizeof(double), c->st));
     {
@@ -546,6 +538,7 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
     Q.d = c->dq.as<double>(); Q.e = c->eq.as<double>();
     Q.lmin = c->lmin.as<double>(); Q.lmax = c->lmax.as<double>();
     Q.lmin_prev = c->lmin_prev.as<double>(); Q.lmax_prev = c->lmax_prev.as<double>();
+    Q.lchg = c->lchg.as<double>();
     Q.count = c->cnt.as<int>(); Q.done = ddone;
     Q.ca = c->ca.as<double>(); Q.ib = c->ib.as<double>(); Q.bprev = c->bprev.as<double>();
     Q.chi = c->chiB.as<double>(); Q.zee = c->zeeB.as<double>();
@@ -569,7 +562,7 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
         launch_zmarch<T, MODE_LANCZOS>(c, B, Z, 1, 4.0 * active * c->M3 * rs);
         passes += active;
         Q.pass = pass;
-        hipLaunchKernelGGL(k_lanczos_scalars, dim3((B + 127) / 128), dim3(128), 0, c->st, Q);
+        hipLaunchKernelGGL(k_lanczos_scalars, dim3((B + 63) / 64), dim3(64), 0, c->st, Q);
         if (pass >= 1) {
             HIPCHK(hipMemcpyAsync(hdone.data(), ddone, (size_t)B * sizeof(int), hipMemcpyDeviceToHost, c->st));
             HIPCHK(hipStreamSynchronize(c->st));

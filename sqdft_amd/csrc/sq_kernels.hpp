@@ -968,17 +968,18 @@ __global__ void __launch_bounds__(NT) k_projs(PArgs<T> A)
 // The projections of k_projs with every image owned by one wave for its
 // whole z-extent.  The workgroup walks the union of its members' cubes in
 // slabs of SZ planes staged in LDS as in k_projs; image li of the z-sorted
-// group list belongs to wave li mod nwave and to that wave's register slot
-// (li / nwave) mod R from the slab where its box enters the union to the
-// slab where it leaves.  Lane l keeps, for each slot, the partial sums of the
+// group list belongs to wave li mod nwave, and the k-th of a wave's images
+// that meets the walk's union holds that wave's register slot k mod R from
+// the slab where its box enters the union to the slab where it leaves.  Lane l keeps, for each slot, the partial sums of the
 // box points whose xy index is l mod 64 over every plane, so the wave reduces
 // (wave_sums) and stores each projection once, when its image leaves: no
 // per-slab reduction, no atomics, no zeroing pass.  Per lane and member the
 // products are added plane by plane, z ascending, in the same round order
 // for every grouping of the nodes, so each member's totals depend only on its
 // own cube.  An image whose box misses the union in z contributes zero.
-// Requires one box size for every image of the list and at most R images of
-// a wave alive at once (k_group_images checks both per group).
+// Requires one box size for every image of the list and R larger than the
+// slot distance of any two images of a wave alive at once (k_group_images
+// computes the R a batch needs; the host picks a compiled R at least that).
 template <typename T, int FORMED, int SZ, int NT, int NM, int R, int RPI>
 __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
 {
@@ -1000,6 +1001,9 @@ __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
     T cab[NM], ibb[NM];
     int nact = 0;
     unsigned onmask = 0;
+    // extent of every member of the group (frozen ones included), so that the
+    // choice between one walk and member-by-member walks is the same in every
+    // pass (k_group_images checks the slot plan of that choice)
     int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
 #pragma unroll
     for (int m = 0; m < NM; ++m) {
@@ -1008,12 +1012,14 @@ __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
         const int node = G.node_list[b0 + min(gs[m], gn - 1)];
         ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
         cab[m] = 0; ibb[m] = 1;
+        if (gs[m] < gn) {
+            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
+            y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
+        }
         if (on[m]) {
             ++nact;
             onmask |= 1u << m;
             if (FORMED) { cab[m] = (T)A.ca[b0 + gs[m]]; ibb[m] = (T)A.ib[b0 + gs[m]]; }
-            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
-            y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
         }
     }
     const int cnt = A.glist_cnt[q];
@@ -1066,8 +1072,9 @@ __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
         const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, upx = upitch_of(ux), up = upx * uy;
         const int np_ = ux * uy, sp = SZ * np_;   // staged points of a slab (unpadded)
         const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
-        // per-wave slot state: started images, live-slot mask, per-slot image data
-        int nstart = 0;
+        // per-wave slot state: images started (slotted or not), slotted
+        // images, live-slot mask, per-slot image data
+        int nstart = 0, nslot = 0;
         unsigned live = 0;
         int s_oz[R], s_ze[R], s_w[R], s_ipb[R], s_t[R];
         long s_uoff[R];
@@ -1128,16 +1135,19 @@ __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
                 if (li >= cnt) break;
                 const int oz = __builtin_amdgcn_readfirstlane(im[li].oz);
                 const int w = __builtin_amdgcn_readfirstlane(im[li].w);
-                const bool miss = oz + w - 1 < uz0 || oz > uz1;
+                const int bx0 = __builtin_amdgcn_readfirstlane(im[li].ox), by0 = __builtin_amdgcn_readfirstlane(im[li].oy);
+                const bool miss = oz + w - 1 < uz0 || oz > uz1 || bx0 + w - 1 < ux0 || bx0 > ux1 ||
+                                  by0 + w - 1 < uy0 || by0 > uy1;
                 const int zs = max(oz, uz0);
                 if (!miss && zs > zl) break;
-                if (miss) {   // no plane of the box in the union: the projection is zero
+                if (miss) {   // the box misses the union: the projection is zero
                     const int ipb = __builtin_amdgcn_readfirstlane(im[li].ipb);
                     if (writer) sw[ipb] = 0.0;
                     ++nstart;
                     continue;
                 }
-                const int slot = nstart % R;
+                const int slot = nslot % R;
+                ++nslot;
                 const int ox = __builtin_amdgcn_readfirstlane(im[li].ox), oy = __builtin_amdgcn_readfirstlane(im[li].oy);
                 const int rlo = max(0, uy0 - oy), rhi = min(w - 1, uy1 - oy);
                 const int clo = max(0, ux0 - ox), chi = min(w - 1, ux1 - ox);
@@ -1455,11 +1465,11 @@ struct LArgs {
     const int *img_type, *img_nn, *nlocc, *nproj;
     int *list, *list_cnt, *overflow;
     int *glist, *glist_cnt;       // [ceil(B/NG)][GLCAP], [ceil(B/NG)]
-    // k_projr eligibility of every group (set to 1 if any fails): one box
-    // size in the list, and at most pr_R live images per wave of pr_nw waves
-    // walking slabs of pr_sz planes
+    // register slots k_projr needs for any group of the batch (a maximum;
+    // 99 when the list mixes box sizes) with pr_nw waves walking slabs of
+    // pr_sz planes, pr_pu the slab capacity per plane and member
     int *pr_bad;
-    int pr_R, pr_nw, pr_sz;
+    int pr_nw, pr_sz, pr_pu;
 };
 
 __global__ void k_node_images(LArgs A, int B)
@@ -1530,29 +1540,58 @@ __global__ void k_group_images(LArgs A, int B)
     }
     A.glist_cnt[q] = cnt;
     if (A.pr_bad && cnt > 0) {
-        // the slot plan of k_projr over the union of all the group's cubes
-        // (any sub-walk clamps to a smaller z-range and only shortens lives)
-        bool bad = false;
+        // the register slots k_projr needs (written as a maximum into
+        // pr_bad; 99 when it does not apply): the k-th image of a wave that
+        // meets the walk's union takes slot k mod R, so R must exceed k - j
+        // for every earlier one j still alive in the slab where k enters.  Over the union of the
+        // group's cubes when k_projr walks them together, else over each
+        // member's cube (a walk with fewer members clamps to a smaller
+        // z-range and only shortens lives).
+        int need = 1;
         const int nc0 = A.nlocc[A.img_type[L[0]]];
-        for (int i = 1; i < cnt; ++i) bad |= A.nlocc[A.img_type[L[i]]] != nc0;
-        const int w = 2 * nc0 + 1, SZ = A.pr_sz, RR = min(A.pr_R, 16);
-        int uz0 = 1 << 30, uz1 = -(1 << 30);
-        for (int m = 0; m < gn; ++m) { uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc); }
-        const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
-        for (int wv = 0; wv < A.pr_nw && !bad; ++wv) {
-            int ring[16];
-            for (int r = 0; r < 16; ++r) ring[r] = -(1 << 30);
-            int k = 0;
-            for (int li = wv; li < cnt && !bad; li += A.pr_nw, ++k) {
-                const int oz = A.img_nn[3 * L[li] + 2] - nc0;
-                if (oz + w - 1 < uz0 || oz > uz1) continue;   // no slot
-                const int S = (max(oz, uz0) - zs0) / SZ, E = (min(oz + w - 1, uz1) - zs0) / SZ;
-                const int slot = k % RR;
-                if (ring[slot] >= S) bad = true;
-                ring[slot] = E;
+        for (int i = 1; i < cnt; ++i)
+            if (A.nlocc[A.img_type[L[i]]] != nc0) need = 99;
+        const int w = 2 * nc0 + 1, SZ = A.pr_sz;
+        int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
+        for (int m = 0; m < gn; ++m) {
+            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]); y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
+        }
+        const int ux = x1 - x0 + 2 * nloc + 1;
+        const int upx = ux + (((w - ux) % 32) + 32) % 32;
+        const bool together = upx * (y1 - y0 + 2 * nloc + 1) <= A.pr_pu;
+        for (int walk = 0; walk < (together ? 1 : gn) && need < 99; ++walk) {
+            int uz0 = 1 << 30, uz1 = -(1 << 30), ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30);
+            for (int m = 0; m < gn; ++m)
+                if (together || m == walk) {
+                    uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc);
+                    ux0 = min(ux0, ni[m] - nloc); ux1 = max(ux1, ni[m] + nloc);
+                    uy0 = min(uy0, nj[m] - nloc); uy1 = max(uy1, nj[m] + nloc);
+                }
+            const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
+            for (int wv = 0; wv < A.pr_nw; ++wv) {
+                constexpr int WN = 32;   // live images looked back on
+                int kk[WN], ee[WN], nl = 0;
+                int k = -1;
+                for (int li = wv; li < cnt; li += A.pr_nw) {
+                    const int oz = A.img_nn[3 * L[li] + 2] - nc0;
+                    const int bx0 = A.img_nn[3 * L[li]] - nc0, by0 = A.img_nn[3 * L[li] + 1] - nc0;
+                    if (oz + w - 1 < uz0 || oz > uz1 || bx0 + w - 1 < ux0 || bx0 > ux1 || by0 + w - 1 < uy0 ||
+                        by0 > uy1)
+                        continue;   // no slot
+                    ++k;
+                    const int S = (max(oz, uz0) - zs0) / SZ, E = (min(oz + w - 1, uz1) - zs0) / SZ;
+                    // drop images that have left before this one enters
+                    int j = 0;
+                    for (int i = 0; i < nl; ++i)
+                        if (ee[i] >= S) { kk[j] = kk[i]; ee[j] = ee[i]; ++j; }
+                    nl = j;
+                    if (nl > 0) need = max(need, k - kk[0] + 1);
+                    if (nl == WN) { need = 99; break; }
+                    kk[nl] = k; ee[nl] = E; ++nl;
+                }
             }
         }
-        if (bad) atomicOr(A.pr_bad, 1);
+        atomicMax(A.pr_bad, need);
     }
 }
 
@@ -1583,6 +1622,7 @@ struct QArgs {
     double *aa, *bb;          // [MAXIT+2][B] (pass-major: the lanes of a wave read one row)
     double *d, *e;            // scratch [B][MAXIT+2]
     double *lmin, *lmax, *lmin_prev, *lmax_prev;
+    double *lchg;             // [B][2] last change of the largest / smallest Ritz value
     int *count, *done;
     double *ca, *ib, *bprev;  // next pass scalars
     double *chi, *zee;        // outputs at convergence
@@ -1641,23 +1681,32 @@ __device__ int ql_extremes(double *d, double *e, int n, double *lmin, double *lm
 }
 
 // Largest and smallest eigenvalue of the symmetric tridiagonal (d[0..n-1],
-// e[0..n-2], strided by st elements) by Newton's method on det(xI - T), from
-// starts xh above and xl below the spectrum.  With q_i = p_i/p_{i-1} (Sturm
-// ratios) and r_i = q_i'/q_i, p'/p = sum_i q_i'/q_i; past an extreme root the
-// characteristic polynomial is monotone and convex (concave on the left), so
-// the iterates move monotonically onto the root and each one stops at the
-// first step that no longer moves inward.  Both iterations share every sweep
-// over the rows, which are read in chunks of CH with the next chunk's loads
-// in flight (the rows live in L2, a few hundred cycles away).  Returns false
-// when either needs more than 200 steps.
-__device__ bool tridiag_extremes(const double *d, const double *e, size_t st, int n, double xh, double xl,
-                                 double *hi, double *lo)
+// e[0..n-2], strided by st elements) by Newton's method on det(xI - T).
+// With q_i = p_i/p_{i-1} (Sturm ratios) and r_i = q_i'/q_i, p'/p = sum_i
+// q_i'/q_i; past an extreme root the characteristic polynomial is monotone
+// and convex (concave on the left), so iterates started outside the spectrum
+// move monotonically onto the root, and each stops at the first step that no
+// longer moves inward.  A start xh is outside (above) the spectrum iff every
+// q_i(xh) > 0 (xh I - T positive definite), xl below iff every q_i(xl) < 0.
+// Each side first tries the near start given (just past the previous pass's
+// extreme, which the new one can only exceed), widening the gap 16-fold per
+// failed sweep, and falls back to the far start (a Weyl bound, always
+// outside).  Both sides share every sweep over the rows, which are read in
+// chunks of CH with the next chunk's loads in flight.  Returns false when
+// either needs more than 200 sweeps.
+__device__ bool tridiag_extremes(const double *d, const double *e, size_t st, int n, double fh, double fl,
+                                 double bh, double bl, double gh, double gl, double *hi, double *lo)
 {
 #pragma clang fp contract(off)
     constexpr int CH = 8;
-    bool dh = false, dl = false;
+    enum { PROBE = 0, NEWTON = 1, DONE = 2 };
+    // each side starts just past the previous extreme (bh + gh, bl - gl) when
+    // that lies inside the far start, else at the far start
+    int mh = bh + gh < fh ? PROBE : NEWTON, ml = bl - gl > fl ? PROBE : NEWTON;
+    double xh = mh == PROBE ? bh + gh : fh, xl = ml == PROBE ? bl - gl : fl;
     for (int it = 0; it < 200; ++it) {
         double qh = xh - d[0], ql = xl - d[0], dqh = 1.0, dql = 1.0;
+        bool ph = qh > 0.0, pl = ql < 0.0;
         double iqh = 1.0 / qh, iql = 1.0 / ql;
         double sh = dqh * iqh, sl = dql * iql;
         double dn[CH], en[CH];
@@ -1686,6 +1735,8 @@ __device__ bool tridiag_extremes(const double *d, const double *e, size_t st, in
                     dql = 1.0 + rl * dql * iql;
                     qh = (xh - dc[j]) - rh;
                     ql = (xl - dc[j]) - rl;
+                    ph = ph && qh > 0.0;
+                    pl = pl && ql < 0.0;
                     iqh = 1.0 / qh;
                     iql = 1.0 / ql;
                     sh += dqh * iqh;
@@ -1693,17 +1744,41 @@ __device__ bool tridiag_extremes(const double *d, const double *e, size_t st, in
                 }
             }
         }
-        if (!dh) {
+        // upper side
+        bool step_h = mh == NEWTON;
+        if (mh == PROBE) {
+            if (ph) {
+                mh = NEWTON;          // outside: Newton from this sweep on
+                step_h = true;
+            } else {
+                gh *= 16.0;           // inside: widen, or take the far start
+                xh = bh + gh;
+                if (!(xh < fh)) { xh = fh; mh = NEWTON; }
+            }
+        }
+        if (step_h) {
             const double xn = xh - 1.0 / sh;
-            if (!(xn < xh)) { *hi = xh; dh = true; }
+            if (!(xn < xh)) { *hi = xh; mh = DONE; }
             else xh = xn;
         }
-        if (!dl) {
+        // lower side
+        bool step_l = ml == NEWTON;
+        if (ml == PROBE) {
+            if (pl) {
+                ml = NEWTON;
+                step_l = true;
+            } else {
+                gl *= 16.0;
+                xl = bl - gl;
+                if (!(xl > fl)) { xl = fl; ml = NEWTON; }
+            }
+        }
+        if (step_l) {
             const double xn = xl - 1.0 / sl;
-            if (!(xn > xl)) { *lo = xl; dl = true; }
+            if (!(xn > xl)) { *lo = xl; ml = DONE; }
             else xl = xn;
         }
-        if (dh && dl) return true;
+        if (mh == DONE && ml == DONE) return true;
     }
     return false;
 }
@@ -1748,7 +1823,12 @@ __global__ void k_lanczos_scalars(QArgs A)
     }
     up += 8.0 * 2.2e-16 * fabs(up) + 1e-300;
     dn -= 8.0 * 2.2e-16 * fabs(dn) + 1e-300;
-    if (!tridiag_extremes(ad, be2, st, nq, up, dn, &hi, &lo)) {
+    // near starts: past the previous extremes by twice their last change
+    // (pass 1 has no previous extremes: the Gershgorin bounds)
+    const double bh = p == 1 ? up : A.lmax_prev[b], bl = p == 1 ? dn : A.lmin_prev[b];
+    const double gh = 2.0 * A.lchg[2 * (size_t)b] + 1e-13 * (fabs(bh) + 1.0);
+    const double gl = 2.0 * A.lchg[2 * (size_t)b + 1] + 1e-13 * (fabs(bl) + 1.0);
+    if (!tridiag_extremes(ad, be2, st, nq, up, dn, bh, bl, gh, gl, &hi, &lo)) {
         double *d = A.d + (size_t)b * S, *e = A.e + (size_t)b * S;
         for (int i = 0; i < nq; ++i) { d[i] = ad[(size_t)i * st]; e[i] = be2[(size_t)i * st]; }
         if (ql_extremes(d, e, nq, &lo, &hi)) { atomicOr(A.err, 1); A.done[b] = 1; return; }
@@ -1756,6 +1836,8 @@ __global__ void k_lanczos_scalars(QArgs A)
     const double dl = fabs(lo - A.lmin_prev[b]), dm = fabs(hi - A.lmax_prev[b]);
     A.lmin_prev[b] = lo;
     A.lmax_prev[b] = hi;
+    A.lchg[2 * (size_t)b] = dm;
+    A.lchg[2 * (size_t)b + 1] = dl;
     const int count = p;   // the reference's count after this iteration
     A.count[b] = count;
     if (!(count < MAXIT && (dl > A.tol || dm > A.tol))) {

@@ -131,6 +131,22 @@ int sqg_node_moments(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
 int sqg_hsub_apply_batch(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
                          const double *v, double *hv);
 
+/* The scratch array of sqg_hsub_apply_batch_noalloc: a file-scope device
+ * array the library holds on every GPU it runs on until the process ends. */
+#define SQG_NOALLOC_SCRATCH_BYTES (64 << 20)
+
+/* sqg_hsub_apply_batch without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  veff (n^3), v and hv
+ * (nb vectors of (2 nloc+1)^3 values, z-major) are FP64 device pointers;
+ * nodes is a host array.  Every per-call buffer (image lists, projections,
+ * the nonlocal term, the FP32 copies of the mixed mode) lives in the
+ * library's SQG_NOALLOC_SCRATCH_BYTES scratch array: SQG_E_OVERFLOW when nb
+ * nodes need more.  Runs on the context's stream and returns after it has
+ * synchronised.  Calls that overlap on one GPU share that array and race. */
+int sqg_hsub_apply_batch_noalloc(sqg_ctx *ctx, const double *veff, const int *nodes, int nb,
+                                 const double *v, double *hv);
+
 /* Device time (ms, HIP events) of the hot kernels over the last density
  * call: launches, summed duration and summed algorithmic bytes (N =
  * (2nloc+1)^3 values per node, s = sizeof(real)).  which: 0 Chebyshev

@@ -118,6 +118,9 @@ SIGNATURES = {
     "sqg_node_moments": (C.c_int, [C.c_void_p, c_dbl_p, c_int_p, C.c_int, c_dbl_p, c_dbl_p, c_dbl_p,
                                    c_int_p]),
     "sqg_hsub_apply_batch": (C.c_int, [C.c_void_p, c_dbl_p, c_int_p, C.c_int, c_dbl_p, c_dbl_p]),
+    # veff, v, hv are device addresses
+    "sqg_hsub_apply_batch_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, c_int_p, C.c_int, C.c_void_p,
+                                               C.c_void_p]),
     "sqg_last_kernel_timing": (C.c_int, [C.c_void_p, C.c_int, c_long_p, c_dbl_p, c_dbl_p]),
     "sqg_density_fixed": (C.c_int, [C.c_void_p, c_dbl_p, C.c_double, C.c_double, C.c_double, c_dbl_p,
                                     c_dbl_p]),

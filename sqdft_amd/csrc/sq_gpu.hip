@@ -92,6 +92,7 @@ struct sqg_ctx {
     DBuf lf_V, lf_Vt, lf_bJ, lf_bJt, lf_img, lf_R, lf_RV, lf_DV, lf_RU, lf_DU, lf_toff, lf_tsize, lf_Z, lf_part, lf_err;
     bool have_state = false;
     double lambda_f = 0.0;
+    unsigned char *scratch_ptr = nullptr;   // this device's copy of `scratch` (sqg_hsub_apply_batch_noalloc)
     // device timing of the hot kernels over the last density call (which:
     // 0 Chebyshev z-march, 1 Lanczos z-march, 2 projections of the Lanczos
     // passes, 3 projections of the Chebyshev steps) and of the whole Lanczos
@@ -121,6 +122,12 @@ struct sqg_ctx {
 };
 
 namespace {
+
+// Per-call buffers of sqg_hsub_apply_batch_noalloc: a file-scope device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (sqg_ctx::scratch_ptr), never through its name.
+__device__ __align__(256) unsigned char scratch[SQG_NOALLOC_SCRATCH_BYTES];
 
 int grid1(size_t n, int bs) { return (int)std::min<size_t>((n + bs - 1) / bs, 1u << 20); }
 
@@ -298,7 +305,7 @@ int node_images(sqg_ctx *c, int B)
     L.glist = c->glist.as<int>();
     L.glist_cnt = c->glist_cnt.as<int>();
     const ProjrShape ps = projr_shape(c);
-    HIPCHK(hipMemsetAsync(c->flags.as<int>() + 4, 0, sizeof(int), c->st));
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, c->st, c->flags.as<int>() + 4, (size_t)1, 0);
     if (ps.nm) {
         L.pr_bad = c->flags.as<int>() + 4;
         L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz; L.pr_pu = projr_pu(c, ps);
@@ -967,6 +974,12 @@ int sqg_create(const sqg_params *p, sqg_ctx **out)
     c->p = *p;
     HIPCHK(hipSetDevice(p->device));
     HIPCHK(hipStreamCreate(&c->st));
+    // this device's address of the allocation-free entry point's scratch array
+    {
+        void *sp = nullptr;
+        HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        c->scratch_ptr = static_cast<unsigned char *>(sp);
+    }
     c->M = 2 * p->nloc + 1;
     c->M2 = c->M * c->M;
     c->M3 = c->M2 * c->M;
@@ -1307,6 +1320,106 @@ int sqg_hsub_apply_batch(sqg_ctx *c, const double *veff, const int *nodes, int n
     HIPCHK(hipMemcpyAsync(hv, tmp.p, len * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return ovf ? SQG_E_OVERFLOW : SQG_OK;
+}
+
+// Points the context's per-batch buffers at regions of the scratch array for
+// the duration of one call and restores them (the context keeps owning its
+// own allocations; nothing here allocates or frees).
+struct ScratchView {
+    sqg_ctx *c;
+    DBuf *bufs[11];
+    void *saved_p[11];
+    size_t saved_b[11];
+    int n = 0;
+    size_t off = 0;
+    explicit ScratchView(sqg_ctx *c_) : c(c_) {}
+    void put(DBuf &d, size_t bytes)
+    {
+        bufs[n] = &d; saved_p[n] = d.p; saved_b[n] = d.bytes; ++n;
+        d.p = c->scratch_ptr + off;
+        d.bytes = bytes;
+        off += (bytes + 255) / 256 * 256;
+    }
+    ~ScratchView()
+    {
+        for (int i = 0; i < n; ++i) { bufs[i]->p = saved_p[i]; bufs[i]->bytes = saved_b[i]; }
+    }
+};
+
+// Scratch bytes the allocation-free H v of nb nodes needs (per-call buffers
+// rounded to 256 bytes each).
+size_t noalloc_bytes(const sqg_ctx *c, int nb)
+{
+    auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t rs = c->real_size(), len = (size_t)nb * c->M3, ng = (size_t)(nb + NG - 1) / NG;
+    size_t t = r(nb * sizeof(int)) + r((size_t)nb * LCAP * sizeof(int)) + r(nb * sizeof(int)) +
+               r(ng * GLCAP * sizeof(int)) + r(ng * sizeof(int)) + r(16 * sizeof(int)) +
+               r((size_t)nb * std::max(c->n_ip, 1) * sizeof(double)) + r(len * rs);
+    if (c->p.precision == SQG_MIXED) t += 2 * r(len * rs) + r((size_t)c->nn * rs);
+    return t;
+}
+
+
+int sqg_hsub_apply_batch_noalloc(sqg_ctx *c, const double *d_veff, const int *nodes, int nb, const double *d_v,
+                                 double *d_hv)
+{
+    if (!c || !d_veff || !nodes || nb < 0 || !d_v || !d_hv) return SQG_E_ARG;
+    if (!c->geom) return SQG_E_NOGEOM;
+    if (!c->scratch_ptr) return SQG_E_HIP;
+    for (int i = 0; i < nb; ++i)
+        if (nodes[i] < 0 || nodes[i] >= c->nn) return SQG_E_ARG;
+    if (nb == 0) return SQG_OK;
+    if (noalloc_bytes(c, nb) > (size_t)SQG_NOALLOC_SCRATCH_BYTES) return SQG_E_OVERFLOW;
+    HIPCHK(hipSetDevice(c->p.device));
+    const size_t len = (size_t)nb * c->M3, rs = c->real_size(), ng = (size_t)(nb + NG - 1) / NG;
+    ScratchView V(c);
+    V.put(c->node_list, nb * sizeof(int));
+    V.put(c->list, (size_t)nb * LCAP * sizeof(int));
+    V.put(c->list_cnt, nb * sizeof(int));
+    V.put(c->glist, ng * GLCAP * sizeof(int));
+    V.put(c->glist_cnt, ng * sizeof(int));
+    V.put(c->flags, 16 * sizeof(int));
+    V.put(c->s, (size_t)nb * std::max(c->n_ip, 1) * sizeof(double));
+    V.put(c->NLv, len * rs);
+    const bool mixed = c->p.precision == SQG_MIXED;
+    if (mixed) {   // FP32 copies of the input, the output and Veff
+        V.put(c->X, len * rs);
+        V.put(c->Y, len * rs);
+        V.put(c->veff, (size_t)c->nn * rs);
+    }
+    // flags start at zero (a fill kernel, not a memset node)
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, c->st, c->flags.as<int>(), (size_t)16, 0);
+    HIPCHK(hipMemcpyAsync(c->node_list.p, nodes, nb * sizeof(int), hipMemcpyHostToDevice, c->st));
+    int st = node_images(c, nb);
+    if (st) return st;
+    const int *dn = c->node_list.as<int>();
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        const T *X;
+        T *Y;
+        if constexpr (sizeof(T) == 8) {
+            X = d_v; Y = d_hv;
+        } else {
+            hipLaunchKernelGGL((k_convert<T>), dim3(grid1(len, 256)), dim3(256), 0, c->st, d_v, c->X.as<T>(), len);
+            hipLaunchKernelGGL((k_convert<T>), dim3(grid1((size_t)c->nn, 256)), dim3(256), 0, c->st, d_veff,
+                               c->veff.as<T>(), (size_t)c->nn);
+            X = c->X.as<T>(); Y = c->Y.as<T>();
+        }
+        PArgs<T> P = pargs<T>(c, dn, nullptr, X);
+        launch_proj<T, 0>(c, nb, P);
+        ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+        if constexpr (sizeof(T) == 8) Z.veff = d_veff;
+        Z.X = const_cast<T *>(X); Z.Y = Y;
+        launch_zmarch<T, MODE_HSUB>(c, nb, Z, -1, 0.0);
+        if constexpr (sizeof(T) == 4)
+            hipLaunchKernelGGL((k_to_double<T>), dim3(grid1(len, 256)), dim3(256), 0, c->st, Y, d_hv, len);
+    };
+    if (mixed) run(float());
+    else run(double());
+    int ovf4[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(ovf4, c->flags.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return (ovf4[0] | ovf4[2] | ovf4[3]) ? SQG_E_OVERFLOW : SQG_OK;
 }
 
 int sqg_forces_nonlocal(sqg_ctx *c, double *f)

@@ -246,6 +246,7 @@ template <typename T, int MODE>
 __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
 {
     constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
+    constexpr bool YQ = MODE == MODE_LANCZOS && sizeof(T) == 8;
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
     const int b = blockIdx.x / ZHALF, h = blockIdx.x % ZHALF;
@@ -329,11 +330,22 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     auto hoff = [&](int q) -> unsigned {
         return (hact && q >= 0 && q < M) ? (unsigned)(q * M2 + hcol) * TS : OOB;
     };
-    // window feed: input value of plane q (LANCZOS forms v = (r' - alpha v_{k-1}) / beta)
-    auto feed = [&](int q) -> T {
+    // window feed of plane q: the raw loads, formed into the input value
+    // (LANCZOS: v = (r' - alpha v_{k-1}) / beta) only where the plane enters
+    // the window, so that no wait for them comes earlier
+    struct FD {
+        T x, y;
+    };
+    auto load_feed = [&](int q) -> FD {
         const unsigned o = voff(q);
-        if (MODE == MODE_LANCZOS) return (bload<T>(rX, o) - ca * bload<T>(rY, o)) * ib;
-        return bload<T>(rX, o);
+        FD f;
+        f.x = bload<T>(rX, o);
+        f.y = MODE == MODE_LANCZOS ? bload<T>(rY, o) : (T)0;
+        return f;
+    };
+    auto formed = [&](const FD &f) -> T {
+        if (MODE == MODE_LANCZOS) return (f.x - ca * f.y) * ib;
+        return f.x;
     };
     struct PW {
         T ve, nl, pv, dv;   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
@@ -344,21 +356,33 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         const int vo = (wrapn(nk - nloc + z, n) * n) * n + gyo + gx;
         P.ve = bload<T>(rV, (act && z < M) ? (unsigned)vo * TS : OOB);
         P.nl = bload<T>(rN, o);
-        P.pv = (cheb_prev || MODE == MODE_LANCZOS) ? bload<T>(rY, o) : (T)0;
+        // FP64 LANCZOS takes v_{k-1} from the feed queue (yq) instead
+        P.pv = (cheb_prev || (MODE == MODE_LANCZOS && !YQ)) ? bload<T>(rY, o) : (T)0;
         P.dv = dacc ? bload<T>(rD, o) : (T)0;
         const unsigned ho = hoff(z);
         P.hx = bload<T>(rX, ho);
         P.hyv = MODE == MODE_LANCZOS ? bload<T>(rY, ho) : (T)0;
     };
 
-    // window: w[s] holds plane z - 6 + s during step z
+    // window: w[s] holds plane z - 6 + s during step z; FP64 LANCZOS: yq[j]
+    // holds the stored v_{k-1} of plane z + 1 + j, read by the feed six steps
+    // before the plane's pointwise inputs are needed (one HBM read of that
+    // vector; FP32 passes measured faster re-reading it from L2)
     T w[WIN];
+    T yq[FDN];
     PW SA, SB;
 #pragma unroll
     for (int s = 0; s < FDN; ++s) w[s] = 0;
-#pragma unroll
-    for (int s = 0; s <= FDN; ++s) w[FDN + s] = feed(s);   // planes 0..6
     fill_pw(0, SA);
+#pragma unroll
+    for (int s = 0; s <= FDN; ++s) {   // planes 0..6
+        const FD f = load_feed(s);
+        w[FDN + s] = formed(f);
+        if (YQ) {
+            if (s == 0) SA.pv = f.y;
+            else yq[s - 1] = f.y;
+        }
+    }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     __syncthreads();
 
@@ -366,8 +390,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     const int hrow = (hy - y0 + FDN) * PP + hx + FDN;      // LDS slot of the halo point
     // step z: S holds plane z's pointwise inputs; N receives plane z + 1's.
     auto step = [&](int z, PW &S, PW &N) {
-        const T qf = feed(z + FDN + 1);
+        const FD qf = load_feed(z + FDN + 1);
         fill_pw(z + 1, N);
+        if (YQ) N.pv = yq[0];
         // stage plane z
         T *pl = lds + (z & 1) * G.plane;
         if (act) pl[lrow] = w[FDN];
@@ -419,7 +444,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         // shift the window and insert plane z + 7
 #pragma unroll
         for (int s = 0; s < WIN - 1; ++s) w[s] = w[s + 1];
-        w[WIN - 1] = qf;
+        w[WIN - 1] = formed(qf);
+        if (YQ) {
+#pragma unroll
+            for (int j = 0; j < FDN - 1; ++j) yq[j] = yq[j + 1];
+            yq[FDN - 1] = qf.y;
+        }
     };
     for (int z = 0; z < M; z += 2) {
         step(z, SA, SB);

@@ -263,16 +263,23 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
 // slots per wave and planes per slab; nm = 0 when it does not apply.
 constexpr int PROJR_NT = 512, PROJR_RPI = 3;
 constexpr int PROJR_SLOTS[3] = {5, 10, 16};   // compiled slot counts (five keep FP64 out of scratch)
-struct ProjrShape { int nm, R, sz; };
+constexpr int PROJM_SLOTS = 16;               // k_projm: one double per slot and lane
+struct ProjrShape { int nm, R, sz, mfma; };
 ProjrShape projr_shape(const sqg_ctx *c)
 {
-    if (c->max_nproj != 1 || c->n_ip == 0) return {0, 0, 0};
+    if (c->max_nproj < 1 || c->max_nproj > 4 || c->n_ip == 0) return {0, 0, 0, 0};
+    // p channels in FP64: k_projm on the matrix cores.  FP32 tables (mixed
+    // precision) stay with k_projs: its VALU sums of FP32 operands ran at
+    // 1.18 ms a pass on the 216-atom LiH batches against k_projm's 1.67 ms
+    // (FP64 MFMAs fed by conversions), while in FP64 the two tie (2.3 ms).
+    if (c->max_nproj > 1 && c->p.precision == SQG_MIXED) return {0, 0, 0, 0};
+    const int mfma = c->max_nproj > 1;
     int w = 0;
     for (int t = 0; t < c->n_typ; ++t)
         if (c->h_nproj[t] > 0) w = std::max(w, 2 * c->h_nlocc[t] + 1);
-    if (w * w > 64 * PROJR_RPI) return {0, 0, 0};
+    if (mfma ? w > 16 : w * w > 64 * PROJR_RPI) return {0, 0, 0, 0};
     // eight members per workgroup; FP32 slabs of two planes fit the LDS
-    return {8, 0, c->p.precision == SQG_MIXED ? 2 : 1};
+    return {8, 0, c->p.precision == SQG_MIXED ? 2 : 1, mfma};
 }
 
 // k_projr's slab capacity per plane and member: the union plane of a 4x2
@@ -284,8 +291,9 @@ int projr_pu(const sqg_ctx *c, const ProjrShape &ps)
     for (int t = 0; t < c->n_typ; ++t)
         if (c->h_nproj[t] > 0) wbox = 2 * c->h_nlocc[t] + 1;
     auto upitch = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
+    auto slab = [&](int u) { return ps.mfma ? u + (((8 - u) % 32) + 32) % 32 : u; };
     const int ux = ps.nm == 8 ? c->M + 3 : c->M + 1, uy = c->M + 1;
-    return std::max(upitch(ux) * uy, upitch(c->M) * c->M);
+    return std::max(slab(upitch(ux) * uy), slab(upitch(c->M) * c->M));
 }
 
 // Image lists of the B nodes in node_list (per node and per group of NG
@@ -308,7 +316,7 @@ int node_images(sqg_ctx *c, int B)
     hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, c->st, c->flags.as<int>() + 4, (size_t)1, 0);
     if (ps.nm) {
         L.pr_bad = c->flags.as<int>() + 4;
-        L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz; L.pr_pu = projr_pu(c, ps);
+        L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz; L.pr_pu = projr_pu(c, ps); L.pr_mfma = ps.mfma;
     }
     hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
     const int ng = (B + NG - 1) / NG;
@@ -322,9 +330,12 @@ int node_images(sqg_ctx *c, int B)
     // the smallest slot count that holds every group of the batch: k_projr
     // gives the same sums with any slot count, so batches stay bitwise alike
     c->projr_R = 0;
-    if (ps.nm)
+    if (ps.nm && ps.mfma) {
+        if (gc[ng] <= PROJM_SLOTS) c->projr_R = PROJM_SLOTS;
+    } else if (ps.nm) {
         for (int r : PROJR_SLOTS)
             if (gc[ng] <= r) { c->projr_R = r; break; }
+    }
     return SQG_OK;
 }
 
@@ -416,6 +427,15 @@ bool launch_projr(sqg_ctx *c, dim3 grid, PArgs<T> &P)
     P.pu = pu;
     constexpr int SZ = sizeof(T) == 8 ? 1 : 2;
     if (ps.sz != SZ) return false;
+    if constexpr (sizeof(T) == 8) {
+        if (ps.mfma) {
+            auto fn = k_projm<T, FORMED, SZ, PROJR_NT, PROJM_SLOTS>;
+            if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            hipLaunchKernelGGL(fn, grid, dim3(PROJR_NT), shm, c->st, P);
+            return true;
+        }
+    }
+    if (ps.mfma) return false;
     if (c->projr_R == 5) projr_go<T, FORMED, SZ, 8, 5>(c, grid, shm, P);
     else if (c->projr_R == 10) projr_go<T, FORMED, SZ, 8, 10>(c, grid, shm, P);
     else projr_go<T, FORMED, SZ, 8, 16>(c, grid, shm, P);

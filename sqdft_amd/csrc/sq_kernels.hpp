@@ -1242,6 +1242,257 @@ __global__ void __launch_bounds__(NT) k_projr(PArgs<T> A)
     }
 }
 
+// The projections of k_projr for tables of up to four projectors (the p
+// channel), accumulated on the matrix cores: every live image of a wave owns
+// one FP64 4x4x4 MFMA accumulator (four 4x4 blocks, one double per lane).
+// Block b = q + 2 ph multiplies members 4q..4q+3 (rows) by the image's
+// projectors (columns) over four box points (K) of phase ph: a box row of
+// side w <= 16 is covered by two MFMAs, point x = 8 hh + 4 ph + k, each into
+// its own accumulator (two independent chains).  The accumulators and the
+// two phases are added when the image leaves the union (lane l + lane l ^ 8),
+// so each total is one fixed sum of MFMA partial sums.  Operands: A = the
+// member's staged value at the point (LDS, member-major, member stride
+// congruent to 8 modulo 32 values so that the 8 members of one point fall on
+// distinct banks), B = the projector value (zero past the box or the
+// union).  A member's row of the product depends only on its own values, so
+// each member's totals are independent of the grouping.  Slot plan, walks and
+// staging as in k_projr.  Lane layout of v_mfma_f64_4x4x4f64 on gfx950:
+// A[b][i][k] at lane 16k + 4b + i, B[b][k][j] at 16k + 4b + j, D[b][i][j] at
+// 16i + 4b + j.
+template <typename T, int FORMED, int SZ, int NT, int R>
+__global__ void __launch_bounds__(NT) k_projm(PArgs<T> A)
+{
+    constexpr int NM = NG;
+    constexpr unsigned TS = sizeof(T);
+    constexpr int PPT = 2;
+    const Geo &G = A.g;
+    const int q = blockIdx.x;
+    const int b0 = q * NG, gn = min(NG, A.B - b0);
+    const int n = G.n, nloc = G.nloc, M = G.M, M2 = G.M2, M3 = G.M3;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int NW = NT / 64;
+    int ni[NM], nj[NM], nk[NM];
+    bool on[NM];
+    T cab[NM], ibb[NM];
+    int nact = 0;
+    unsigned onmask = 0;
+    int x0 = 1 << 30, x1 = -(1 << 30), y0 = 1 << 30, y1 = -(1 << 30);
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        on[m] = m < gn && !(G.done && G.done[b0 + m]);
+        const int node = G.node_list[b0 + min(m, gn - 1)];
+        ni[m] = node % n; nj[m] = (node / n) % n; nk[m] = node / (n * n);
+        cab[m] = 0; ibb[m] = 1;
+        if (m < gn) {
+            x0 = min(x0, ni[m]); x1 = max(x1, ni[m]);
+            y0 = min(y0, nj[m]); y1 = max(y1, nj[m]);
+        }
+        if (on[m]) {
+            ++nact;
+            onmask |= 1u << m;
+            if (FORMED) { cab[m] = (T)A.ca[b0 + m]; ibb[m] = (T)A.ib[b0 + m]; }
+        }
+    }
+    const int cnt = A.glist_cnt[q];
+    if (nact == 0 || cnt == 0) return;
+    // LDS: image data [cnt] | staged slab [SZ][NM][member stride]
+    extern __shared__ unsigned char smem_raw[];
+    ImgMeta *im = reinterpret_cast<ImgMeta *>(smem_raw);
+    T *pl = reinterpret_cast<T *>(smem_raw + ((size_t)cnt * sizeof(ImgMeta) + 15) / 16 * 16);
+    for (int li = tid; li < cnt; li += blockDim.x) {
+        const int J = A.glist[(size_t)q * GLCAP + li];
+        const int t = A.img_type[J], nc = A.nlocc[t];
+        ImgMeta e;
+        e.ox = A.img_nn[3 * J] - nc; e.oy = A.img_nn[3 * J + 1] - nc; e.oz = A.img_nn[3 * J + 2] - nc;
+        e.w = 2 * nc + 1; e.np = A.nproj[t]; e.t = t; e.ipb = A.ip_base[J];
+        e.uoff = A.udv_off[J];
+        im[li] = e;
+    }
+    __syncthreads();
+    const size_t gb = (size_t)gn * M3 * TS;
+    const __amdgpu_buffer_rsrc_t rX = brsrc(A.V + (size_t)b0 * M3, gb);
+    const __amdgpu_buffer_rsrc_t rY = brsrc((FORMED ? A.V2 : A.V) + (size_t)b0 * M3, gb);
+    const int wbox = __builtin_amdgcn_readfirstlane(im[0].w);
+    auto upitch_of = [&](int u) { return u + (((wbox - u) % 32) + 32) % 32; };
+    auto mstride_of = [&](int u) { return u + (((8 - u) % 32) + 32) % 32; };
+    const bool together = mstride_of(upitch_of(x1 - x0 + M) * (y1 - y0 + M)) <= A.pu;
+    // lane roles in the MFMA: K index, block (member quad, point phase), row/column
+    const int kk = lane >> 4, bb = (lane >> 2) & 3, ii = lane & 3;
+    const int mA = 4 * (bb & 1) + ii;          // member of the lane's A value
+    const int bxl0 = 4 * (bb >> 1) + kk;       // box x of the lane's point, first MFMA of a row
+    // writers of the totals: lanes of phase 0; member 4q + i, projector j
+    const int mw = 4 * ((lane >> 2) & 1) + (lane >> 4), jw = lane & 3;
+    const bool wl = (lane & 8) == 0;
+    double *sw = A.s + (size_t)(b0 + mw) * A.n_ip;
+#pragma unroll 1
+    for (int sub = 0; sub < (together ? 1 : NM); ++sub) {
+        bool in_sub[NM];
+        unsigned submask = 0;
+        int ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30), uz0 = 1 << 30, uz1 = -(1 << 30);
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            in_sub[m] = on[m] && (together || m == sub);
+            if (in_sub[m]) {
+                submask |= 1u << m;
+                ux0 = min(ux0, ni[m] - nloc); ux1 = max(ux1, ni[m] + nloc);
+                uy0 = min(uy0, nj[m] - nloc); uy1 = max(uy1, nj[m] + nloc);
+                uz0 = min(uz0, nk[m] - nloc); uz1 = max(uz1, nk[m] + nloc);
+            }
+        }
+        if (ux1 < ux0) continue;
+        const bool writer = wl && ((submask >> mw) & 1u);
+        const int ux = ux1 - ux0 + 1, uy = uy1 - uy0 + 1, upx = upitch_of(ux), ms = mstride_of(upx * uy);
+        const int np_ = ux * uy, sp = SZ * np_;
+        const int zs0 = (uz0 + 4096) / SZ * SZ - 4096;
+        int nstart = 0, nslot = 0;
+        unsigned live = 0;
+        int s_oz[R], s_ze[R], s_ipb[R], s_t[R], s_np[R], s_ox[R], s_oy[R];
+        long s_uoff[R];
+        double cacc[R][2];   // per slot: the first and second MFMA of each box row
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            s_oz[r] = s_ze[r] = s_ipb[r] = s_t[r] = s_np[r] = s_ox[r] = s_oy[r] = 0;
+            s_uoff[r] = 0;
+            cacc[r][0] = cacc[r][1] = 0.0;
+        }
+#pragma unroll 1
+        for (int gzs = zs0; gzs <= uz1; gzs += SZ) {
+            __syncthreads();
+#pragma unroll 1
+            for (int p0 = 0; p0 < sp; p0 += PPT * (int)blockDim.x) {
+                T xv[PPT][NM], yv[PPT][NM];
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    const int sl = p / np_, pq = p - sl * np_;
+                    const int yy = pq / ux, xx = pq - yy * ux;
+                    const int gx = ux0 + xx, gy = uy0 + yy, gz = gzs + sl;
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) {
+                        const int lx = gx - ni[m] + nloc, ly = gy - nj[m] + nloc, lz = gz - nk[m] + nloc;
+                        const bool ok = p < sp && in_sub[m] && (unsigned)lx < (unsigned)M &&
+                                        (unsigned)ly < (unsigned)M && (unsigned)lz < (unsigned)M;
+                        const unsigned o = ok ? (unsigned)(m * M3 + lz * M2 + ly * M + lx) * TS : OOB;
+                        xv[r][m] = bload<T>(rX, o);
+                        yv[r][m] = FORMED ? bload<T>(rY, o) : (T)0;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int p = p0 + r * (int)blockDim.x + tid;
+                    if (p < sp) {
+                        const int sl = p / np_, pq = p - sl * np_;
+                        const int yy = pq / ux, xx = pq - yy * ux;
+                        T *dst = pl + (size_t)sl * NM * ms + yy * upx + xx;
+#pragma unroll
+                        for (int m = 0; m < NM; ++m)
+                            dst[(size_t)m * ms] = FORMED ? (xv[r][m] - cab[m] * yv[r][m]) * ibb[m] : xv[r][m];
+                    }
+                }
+            }
+            __syncthreads();
+            const int zl = gzs + SZ - 1;
+#pragma unroll 1
+            for (;;) {
+                const int li = wave + NW * nstart;
+                if (li >= cnt) break;
+                const int oz = __builtin_amdgcn_readfirstlane(im[li].oz);
+                const int w = wbox;
+                const int bx0 = __builtin_amdgcn_readfirstlane(im[li].ox), by0 = __builtin_amdgcn_readfirstlane(im[li].oy);
+                const bool miss = oz + w - 1 < uz0 || oz > uz1 || bx0 + w - 1 < ux0 || bx0 > ux1 ||
+                                  by0 + w - 1 < uy0 || by0 > uy1;
+                const int zs = max(oz, uz0);
+                if (!miss && zs > zl) break;
+                const int ipb = __builtin_amdgcn_readfirstlane(im[li].ipb);
+                const int np = __builtin_amdgcn_readfirstlane(im[li].np);
+                if (miss) {   // the box misses the union: the projections are zero
+                    if (writer && jw < np) sw[ipb + jw] = 0.0;
+                    ++nstart;
+                    continue;
+                }
+                const int slot = nslot % R;
+                ++nslot;
+                const int t = __builtin_amdgcn_readfirstlane(im[li].t);
+                const long uoff = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(im[li].uoff >> 32)) << 32) |
+                                         (unsigned)__builtin_amdgcn_readfirstlane((int)im[li].uoff));
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r == slot) {
+                        s_oz[r] = oz; s_ze[r] = min(oz + w - 1, uz1); s_ipb[r] = ipb; s_t[r] = t; s_np[r] = np;
+                        s_ox[r] = bx0; s_oy[r] = by0; s_uoff[r] = uoff;
+                        cacc[r][0] = cacc[r][1] = 0.0;
+                    }
+                }
+                live |= 1u << slot;
+                ++nstart;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!((live >> r) & 1u)) continue;
+                const int w = wbox, w2 = w * w, w3 = w2 * w;
+                const int ox = s_ox[r], oy = s_oy[r];
+                const int za = max(gzs, max(s_oz[r], uz0)), zb = min(zl, s_ze[r]);
+                const int rlo = max(0, uy0 - oy), rhi = min(w - 1, uy1 - oy);
+                const int clo = max(0, ux0 - ox), chi = min(w - 1, ux1 - ox);
+                const __amdgpu_buffer_rsrc_t rU = brsrc(A.udv + s_uoff[r], (size_t)s_np[r] * w3 * TS);
+                const bool colj = ii < s_np[r];
+                bool okx[2];
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) okx[hh] = bxl0 + 8 * hh >= clo && bxl0 + 8 * hh <= chi;
+                for (int z = za; z <= zb; ++z) {
+                    const int bz = z - s_oz[r];
+                    const T *pz = pl + (size_t)(z - gzs) * NM * ms + (size_t)mA * ms;
+                    // rows in chunks of RC, the next chunk's operands loaded
+                    // while the current chunk's MFMAs run
+                    constexpr int RC = 4;
+                    T an[RC][2], bn[RC][2];
+                    auto load_rows = [&](int by0) {
+#pragma unroll
+                        for (int j = 0; j < RC; ++j) {
+                            const int by = by0 + j;
+                            const bool rowok = by <= rhi;
+                            const int lrow = (oy + by - uy0) * upx + (ox - ux0);
+                            const int urow = ii * w3 + bz * w2 + by * w;
+#pragma unroll
+                            for (int hh = 0; hh < 2; ++hh) {
+                                const int bx = bxl0 + 8 * hh;
+                                const bool ok = rowok && okx[hh];
+                                an[j][hh] = ok ? pz[lrow + bx] : (T)0;
+                                bn[j][hh] = bload<T>(rU, (ok && colj) ? (unsigned)(urow + bx) * TS : OOB);
+                            }
+                        }
+                    };
+                    load_rows(rlo);
+                    for (int by0 = rlo; by0 <= rhi; by0 += RC) {
+                        T ac[RC][2], bc[RC][2];
+#pragma unroll
+                        for (int j = 0; j < RC; ++j)
+#pragma unroll
+                            for (int hh = 0; hh < 2; ++hh) { ac[j][hh] = an[j][hh]; bc[j][hh] = bn[j][hh]; }
+                        if (by0 + RC <= rhi) load_rows(by0 + RC);
+#pragma unroll
+                        for (int j = 0; j < RC; ++j) {
+                            if (by0 + j <= rhi) {
+#pragma unroll
+                                for (int hh = 0; hh < 2; ++hh)
+                                    cacc[r][hh] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)ac[j][hh], (double)bc[j][hh],
+                                                                                     cacc[r][hh], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+                if (s_ze[r] <= zl) {   // the image leaves the union: add the partial sums and store
+                    const double c = cacc[r][0] + cacc[r][1];
+                    const double tot = c + __shfl_xor(c, 8, 64);
+                    if (writer && jw < s_np[r]) sw[s_ipb[r] + jw] = tot * A.inv_denom[16 * s_t[r] + jw];
+                    live &= ~(1u << r);
+                }
+            }
+        }
+    }
+}
+
 // The nonlocal term of k_nlapply for the NG consecutive slots of a group:
 // each point of the union of the members' cubes reads its ELL entries once
 // and applies them to every member's projections, which sit in LDS
@@ -1500,6 +1751,7 @@ struct LArgs {
     // pr_sz planes, pr_pu the slab capacity per plane and member
     int *pr_bad;
     int pr_nw, pr_sz, pr_pu;
+    int pr_mfma;                  // k_projm's member stride (congruent to 8 modulo 32)
 };
 
 __global__ void k_node_images(LArgs A, int B)
@@ -1588,7 +1840,9 @@ __global__ void k_group_images(LArgs A, int B)
         }
         const int ux = x1 - x0 + 2 * nloc + 1;
         const int upx = ux + (((w - ux) % 32) + 32) % 32;
-        const bool together = upx * (y1 - y0 + 2 * nloc + 1) <= A.pr_pu;
+        int slab = upx * (y1 - y0 + 2 * nloc + 1);
+        if (A.pr_mfma) slab += (((8 - slab) % 32) + 32) % 32;
+        const bool together = slab <= A.pr_pu;
         for (int walk = 0; walk < (together ? 1 : gn) && need < 99; ++walk) {
             int uz0 = 1 << 30, uz1 = -(1 << 30), ux0 = 1 << 30, ux1 = -(1 << 30), uy0 = 1 << 30, uy1 = -(1 << 30);
             for (int m = 0; m < gn; ++m)

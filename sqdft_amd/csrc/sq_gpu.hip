@@ -499,7 +499,7 @@ apply:
 // LDS bytes of one z-march workgroup: two staged planes and the reduction scratch
 size_t zmarch_lds(const sqg_ctx *c)
 {
-    return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double);
+    return 2 * (size_t)c->plane * sizeof(double) + 16 * 4 * sizeof(double);
 }
 
 template <typename T, int MODE, int LB>
@@ -521,8 +521,9 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         if (e0) (void)hipEventRecord(e0, c->st);
     }
     // FP32 workgroups of up to 640 threads (M <= 35) are held to 96 VGPRs so
-    // that two share a CU; FP64 ones would spill there and run one per CU
-    if (sizeof(T) == 4 && zmarch_threads(c->M) <= 640) zmarch_k<T, MODE, 640>(c, B, A, shm);
+    // that two share a CU; FP64 ones (and the FP32 force pass, with its D
+    // vector) would spill there and run one per CU
+    if (sizeof(T) == 4 && MODE != MODE_CHEBD && zmarch_threads(c->M) <= 640) zmarch_k<T, MODE, 640>(c, B, A, shm);
     else zmarch_k<T, MODE, 1024>(c, B, A, shm);
     if (which >= 0 && e0 && e1) {
         (void)hipEventRecord(e1, c->st);
@@ -633,17 +634,19 @@ int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
     // ---------------- Chebyshev moments (sq.cpp:564-655)
     const int c_idx = (c->p.nloc * c->M + c->p.nloc) * c->M + c->p.nloc;
     hipLaunchKernelGGL((k_fill_delta<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, c->M3, B, c_idx);
-    T *bufs[2] = {X, Y};
+    // t_j lives in bufs[j % 3]: step k reads t_{k-1} and t_{k-2} and writes
+    // t_k into the third buffer (no vector is overwritten while it is read)
+    T *bufs[3] = {X, Y, c->X2.as<T>()};
     MArgs Mg{};
     Mg.B = B; Mg.npl = c->p.npl; Mg.h3 = std::pow(c->p.h, 3);
     Mg.node_list = dn; Mg.center = c->center.as<double>(); Mg.dots = c->dots.as<double>();
     Mg.htemp = c->htemp.as<double>(); Mg.mu = c->mu_all.as<double>();
     for (int k = 1; k <= c->p.npl / 2; ++k) {
-        T *in = bufs[(k - 1) & 1], *out = bufs[k & 1];
+        T *in = bufs[(k - 1) % 3], *prev = bufs[(k + 1) % 3], *out = bufs[k % 3];
         PArgs<T> P = pargs<T>(c, dn, nullptr, in);
         launch_proj<T, 0>(c, B, P, 3, (double)B * c->M3 * rs);   // reads t_k
         ZArgs<T> Z = zargs<T>(c, dn, nullptr);
-        Z.X = in; Z.Y = out;
+        Z.X = in; Z.Y = prev; Z.O = out;
         Z.chi = c->chiB.as<double>(); Z.zee = c->zeeB.as<double>();
         Z.first = k == 1;
         Z.center = c->center.as<double>(); Z.dots = c->dots.as<double>();
@@ -876,13 +879,15 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
     T *X = c->X.as<T>(), *Y = c->Y.as<T>(), *D = c->Dm.as<T>();
     const int c_idx = (c->p.nloc * c->M + c->p.nloc) * c->M + c->p.nloc;
     hipLaunchKernelGGL((k_fill_delta<T>), dim3(grid1(vlen, 256)), dim3(256), 0, c->st, X, c->M3, B, c_idx);
-    T *bufs[2] = {X, Y};
+    // t_j lives in bufs[j % 3]: step k reads t_{k-1} and t_{k-2} and writes
+    // t_k into the third buffer (no vector is overwritten while it is read)
+    T *bufs[3] = {X, Y, c->X2.as<T>()};
     for (int k = 1; k <= c->p.npl; ++k) {
-        T *in = bufs[(k - 1) & 1], *out = bufs[k & 1];
+        T *in = bufs[(k - 1) % 3], *prev = bufs[(k + 1) % 3], *out = bufs[k % 3];
         PArgs<T> P = pargs<T>(c, dn, nullptr, in);
         launch_proj<T, 0>(c, B, P);
         ZArgs<T> Z = zargs<T>(c, dn, nullptr);
-        Z.X = in; Z.Y = out;
+        Z.X = in; Z.Y = prev; Z.O = out;
         Z.chi = c->chi_all.as<double>(); Z.zee = c->zee_all.as<double>(); Z.chi_by_node = 1;
         Z.first = k == 1;
         Z.center = c->center.as<double>(); Z.dots = c->dots.as<double>();

@@ -265,12 +265,17 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     const int hy = h == 0 ? y0 + ny + hr : y0 - FDN + hr;
     const bool hact = hr < FDN && hy >= 0 && hy < M;
 
-    // LDS: two staged planes (rows y0 - 6 .. y0 + ny + 5) | reduction scratch
+    // LDS: two staged planes (rows y0 - 6 .. y0 + ny + 5) | reduction scratch.
+    // Chebyshev steps hold the planes and the register window in FP64 in both
+    // modes (stored values converted once, where they enter, so the stencil
+    // reads need no conversions); FP32 Lanczos passes keep FP32 (fewer
+    // registers, measured faster).
+    typedef std::conditional_t<CH, double, T> W;
     extern __shared__ unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(T));
+    W *lds = reinterpret_cast<W *>(smem_raw);
+    double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(double));
     __shared__ double sh_center;
-    for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (T)0;
+    for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (W)0;
     if (tid == 0) sh_center = 0.0;
 
     // buffer resources of this node's vectors and of Veff
@@ -368,7 +373,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     // holds the stored v_{k-1} of plane z + 1 + j, read by the feed six steps
     // before the plane's pointwise inputs are needed (one HBM read of that
     // vector; FP32 passes measured faster re-reading it from L2)
-    T w[WIN];
+    W w[WIN];
     T yq[FDN];
     PW SA, SB;
 #pragma unroll
@@ -377,7 +382,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
 #pragma unroll
     for (int s = 0; s <= FDN; ++s) {   // planes 0..6
         const FD f = load_feed(s);
-        w[FDN + s] = formed(f);
+        w[FDN + s] = (W)formed(f);
         if (YQ) {
             if (s == 0) SA.pv = f.y;
             else yq[s - 1] = f.y;
@@ -394,12 +399,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         fill_pw(z + 1, N);
         if (YQ) N.pv = yq[0];
         // stage plane z
-        T *pl = lds + (z & 1) * G.plane;
+        W *pl = lds + (z & 1) * G.plane;
         if (act) pl[lrow] = w[FDN];
-        if (hact) pl[hrow] = MODE == MODE_LANCZOS ? (S.hx - ca * S.hyv) * ib : S.hx;
+        if (hact) pl[hrow] = (W)(MODE == MODE_LANCZOS ? (T)((S.hx - ca * S.hyv) * ib) : S.hx);
         lds_barrier();
         if (act) {
-            const T *row = pl + lrow;
+            const W *row = pl + lrow;
             const R cv = (R)w[FDN];
             R t = cv * c0d;
 #pragma unroll
@@ -444,7 +449,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         // shift the window and insert plane z + 7
 #pragma unroll
         for (int s = 0; s < WIN - 1; ++s) w[s] = w[s + 1];
-        w[WIN - 1] = formed(qf);
+        w[WIN - 1] = (W)formed(qf);
         if (YQ) {
 #pragma unroll
             for (int j = 0; j < FDN - 1; ++j) yq[j] = yq[j + 1];

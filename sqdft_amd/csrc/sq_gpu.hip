@@ -499,15 +499,25 @@ apply:
 // LDS bytes of one z-march workgroup: two staged planes and the reduction scratch
 size_t zmarch_lds(const sqg_ctx *c)
 {
-    return 2 * (size_t)c->plane * sizeof(double) + 16 * 4 * sizeof(double);
+    return 2 * (size_t)c->plane * c->real_size() + 16 * 4 * sizeof(double);
 }
 
 template <typename T, int MODE, int LB>
 void zmarch_k(sqg_ctx *c, int B, const ZArgs<T> &A, size_t shm)
 {
-    auto fn = LB <= 640 ? k_zmarch<T, MODE> : k_zmarch_big<T, MODE>;
+    void (*fn)(ZArgs<T>);
+    if constexpr (LB <= 640) fn = k_zmarch<T, MODE>;
+    else fn = k_zmarch_big<T, MODE>;
     if (shm > 64 * 1024) (void)hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     hipLaunchKernelGGL(fn, dim3(B * ZHALF), dim3(zmarch_threads(c->M)), shm, c->st, A);
+}
+
+void finish_zmarch(sqg_ctx *c, int which, hipEvent_t e0, hipEvent_t e1, double alg_bytes)
+{
+    if (which >= 0 && e0 && e1) {
+        (void)hipEventRecord(e1, c->st);
+        c->ev[which].push_back({e0, e1, alg_bytes});
+    }
 }
 
 template <typename T, int MODE>
@@ -520,15 +530,17 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         e1 = c->ev_get();
         if (e0) (void)hipEventRecord(e0, c->st);
     }
-    // FP32 workgroups of up to 640 threads (M <= 35) are held to 96 VGPRs so
+    // FP32 workgroups of up to 640 threads (M <= 35) are held to 80 VGPRs so
     // that two share a CU; FP64 ones (and the FP32 force pass, with its D
     // vector) would spill there and run one per CU
-    if (sizeof(T) == 4 && MODE != MODE_CHEBD && zmarch_threads(c->M) <= 640) zmarch_k<T, MODE, 640>(c, B, A, shm);
-    else zmarch_k<T, MODE, 1024>(c, B, A, shm);
-    if (which >= 0 && e0 && e1) {
-        (void)hipEventRecord(e1, c->st);
-        c->ev[which].push_back({e0, e1, alg_bytes});
+    if constexpr (sizeof(T) == 4 && MODE != MODE_CHEBD) {
+        if (zmarch_threads(c->M) <= 640) {
+            zmarch_k<T, MODE, 640>(c, B, A, shm);
+            return finish_zmarch(c, which, e0, e1, alg_bytes);
+        }
     }
+    zmarch_k<T, MODE, 1024>(c, B, A, shm);
+    finish_zmarch(c, which, e0, e1, alg_bytes);
 }
 
 // Lanczos bounds of the B nodes in node_list (image lists already built).

@@ -266,14 +266,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     const bool hact = hr < FDN && hy >= 0 && hy < M;
 
     // LDS: two staged planes (rows y0 - 6 .. y0 + ny + 5) | reduction scratch.
-    // Chebyshev steps hold the planes and the register window in FP64 in both
-    // modes (stored values converted once, where they enter, so the stencil
-    // reads need no conversions); FP32 Lanczos passes keep FP32 (fewer
-    // registers, measured faster).
-    typedef std::conditional_t<CH, double, T> W;
+    // The planes and the register window hold the stored type (FP32 in the
+    // mixed mode: fewer registers, two workgroups per CU).
+    typedef T W;
     extern __shared__ unsigned char smem_raw[];
     W *lds = reinterpret_cast<W *>(smem_raw);
-    double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(double));
+    double *red = reinterpret_cast<double *>(smem_raw + 2 * (size_t)G.plane * sizeof(W));
     __shared__ double sh_center;
     for (int i = tid; i < 2 * G.plane; i += blockDim.x) lds[i] = (W)0;
     if (tid == 0) sh_center = 0.0;
@@ -475,10 +473,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     }
 }
 
-// Up to 640 threads (M <= 35): at most 96 VGPRs, so that two workgroups
-// (20 waves) share a CU; larger cubes take up to 1024 threads, one per CU.
+// FP32, up to 640 threads (M <= 35): at most 80 VGPRs, six waves per SIMD,
+// so that two 10-wave workgroups share a CU whatever SIMDs their waves land
+// on (at 96 VGPRs, five per SIMD, the second one does not always fit).
+// Otherwise up to 1024 threads and no register cap.
 template <typename T, int MODE>
-__global__ __attribute__((amdgpu_waves_per_eu(5, 8))) void __launch_bounds__(640) k_zmarch(ZArgs<T> A)
+__global__ __attribute__((amdgpu_waves_per_eu(6, 8))) void __launch_bounds__(640) k_zmarch(ZArgs<T> A)
 {
     zmarch_body<T, MODE>(A);
 }

@@ -272,6 +272,7 @@ def main():
             out["full_step"]["mixed"] = {
                 "s": t32, "nodes_per_s": nn / t32, "speedup_vs_fp64": t64 / t32,
                 "rho_rel_err_vs_fp64": float(np.max(np.abs(d32["rho"] - d64["rho"])) / np.max(np.abs(d64["rho"]))),
+                "rho_rel_err_l2_vs_fp64": float(np.linalg.norm(d32["rho"] - d64["rho"]) / np.linalg.norm(d64["rho"])),
                 "lambda_f_err": abs(d32["lambda_f"] - d64["lambda_f"]),
                 "ebs_rel_err": abs(d32["ebs"] - d64["ebs"]) / abs(d64["ebs"])}
     if not args.no_scf:

@@ -85,7 +85,7 @@ def test_density_matches_oracle(cases, name):
     assert np.array_equal(nd["iters"], ref["iters"])
     assert H.rel(nd["chi"], ref["chi"]) < FP64_TOL
     assert H.rel(nd["mu"], ref["mu"]) < FP64_TOL
-    assert H.rel(nd["Ci"], ref["Ci"]) < 1e-9
+    assert H.rel(nd["Ci"], ref["Ci"]) < FP64_TOL
     assert out["stats"]["fermi_iters"] == ref["fermi_iters"]
     # second SCF iteration: bracket lambda_f +- 2 (sq.cpp:1301-1304)
     ref2 = osys.density(veff + 0.01, scf_iter=2, lambda_f=ref["lambda_f"], nthreads=0)

@@ -37,14 +37,13 @@ def test_poisson_and_vxc_match_oracle(al):
     g = scf.PoissonSolver_AAJ(rhs, np.ones(n ** 3))
     git, grr = scf.poisson_log[-1]
     o, oit, orr = O.poisson_aaj(n, F, lap, rhs, np.ones(n ** 3), pp["tol"], pp["maxiter"], pp["beta"], pp["m"], pp["p"])
-    # same iterations; the Anderson steps solve a 7x7 Gram system whose
-    # condition number amplifies the different summation order of the
-    # device reductions, so the iterates agree to ~1e-9, not to 1e-15
+    # same iterations and iterates (measured 1.3e-13 apart: the device
+    # reductions sum in another order than the oracle's loops)
     dphi = np.max(np.abs(g - o)) / np.max(np.abs(o))
     say("poisson: its", git, oit, "relres", grr, orr, "max rel phi diff", dphi)
     assert git == oit
     assert abs(grr - orr) <= 1e-6 * orr
-    assert dphi <= 1e-9
+    assert dphi <= 1e-10
     v = scf.ExchangeCorrelationPotential(scf.rho)
     ov = O.vxc_lda(scf.rho)
     assert np.max(np.abs(v - ov)) <= 1e-13 * np.max(np.abs(ov))

@@ -14,6 +14,13 @@ from the electronic one (src/md.cpp:385-386); ``T_ion`` overrides it.
 
 Units: bohr, fs, Ha; masses in amu are converted with 1 amu =
 1.066572305 Ha fs^2 / bohr^2 (src/md.cpp:389).
+
+Files (``sqdft_amd.files``): ``restart_md`` continues from a .restartMD
+file (RestartMD, src/main.cpp:91-100: step count, velocities, forces ->
+accelerations, positions, energy sums); ``prnt_md`` writes one after every
+step (PrintMD, src/main.cpp:139-140); ``trajectory`` appends the positions
+of every step (datacollect, src/md.cpp:76-77); ``vout`` the atom speeds
+(PrintVels).
 """
 from __future__ import annotations
 
@@ -22,6 +29,7 @@ import time
 
 import numpy as np
 
+from . import files
 from .scf import SCF, pinv_times_vec
 
 AMU = 1.066572305                     # Ha fs^2 / bohr^2 per amu
@@ -97,7 +105,9 @@ class NVE:
     place).  masses: amu per atom type."""
 
     def __init__(self, scf: SCF, masses, time_step: float = 1.0, T_ion: float | None = None,
-                 rand_seed: int = 2, vds: int = 1, chg_extrap: bool = True, **scf_kw):
+                 rand_seed: int = 2, vds: int = 1, chg_extrap: bool = True, restart_md: str | None = None,
+                 prnt_md: str | None = None, trajectory: str | None = None, vout: str | None = None,
+                 max_steps: int = 0, **scf_kw):
         self.scf = scf
         S = scf.S
         xyz, typ = S.atoms()
@@ -121,6 +131,19 @@ class NVE:
         self.drho_dt, self.drho_2dt, self.drho_new = np.zeros(n3), np.zeros(n3), np.zeros(n3)
         self.r0, self.r1, self.r2 = np.zeros_like(self.x), np.zeros_like(self.x), np.zeros_like(self.x)
         self.history = []
+        self.prnt_md, self.trajectory, self.vout, self.max_steps = prnt_md, trajectory, vout, max_steps
+        self.restart_count = 0
+        self.sums = np.zeros(8)                   # TE, TE^2, PE, PE^2, KE, KE^2, T, T^2 over all steps
+        if restart_md:
+            st = files.read_restart_md(restart_md, self.na)
+            if max_steps and st.step_count >= max_steps:
+                raise ValueError(f"{restart_md}: {st.step_count} of {max_steps} MD steps already done")
+            self.restart_count = st.step_count
+            self.v = st.vels.copy()
+            self.a = st.forces / self.m[:, None]
+            self.x = st.xyz.copy()
+            self.x_nm = st.xyz.copy()
+            self.sums = st.sums.copy()
 
     def step(self) -> dict:
         t0 = time.perf_counter()
@@ -169,7 +192,28 @@ class NVE:
                "positions": self.x.copy(),
                "seconds": time.perf_counter() - t0}
         self.history.append(rec)
+        for i, q in enumerate((rec["TE"], rec["PE"], rec["KE"], rec["T"])):
+            self.sums[2 * i] += q
+            self.sums[2 * i + 1] += q * q
+        total = k + self.restart_count
+        rec["md_step"] = total
+        if self.trajectory:
+            files.append_trajectory(self.trajectory, k, self.max_steps, [self.L] * 3, self.x)
+        if self.vout:
+            files.write_vout(self.vout, k, self.v)
+        if self.prnt_md:
+            files.write_restart_md(self.prnt_md, files.MDState(total, self.v, f, self.x, self.sums))
         return rec
+
+    def means(self) -> dict:
+        """Mean and standard deviation of TE, PE, KE and T over every step so
+        far, restarted runs included (src/main.cpp:144-148)."""
+        n = max(self.step_count + self.restart_count, 1)
+        out = {}
+        for i, name in enumerate(("TE", "PE", "KE", "T")):
+            m = self.sums[2 * i] / n
+            out[name] = (m, float(np.sqrt(abs(self.sums[2 * i + 1] / n - m * m))))
+        return out
 
     def run(self, steps: int) -> list:
         return [self.step() for _ in range(steps)]

@@ -93,6 +93,7 @@ class SCF:
         self.veff = None
         self.scf_iter = 0
         self.poisson_log = []   # (iterations, relative residual) of every solve
+        self.prnt_scf = None    # path: the density of every SCF iteration goes there (.restart layout)
 
     def close(self):
         self.sq.close()
@@ -218,6 +219,12 @@ class SCF:
         self.veff = None
         self.scf_iter = 0
 
+    def restart_density(self, path: str, nproc: int | None = None) -> None:
+        """Starting density from a .restart file (RestartSCF,
+        src/initialize.cpp:675-678), scaled like any guess."""
+        from . import files
+        self.set_density_guess(files.read_restart(path, self.S.n, nproc), keep_phi=False)
+
     def set_density_guess(self, rho, keep_phi: bool = True) -> None:
         """Starting density: rho scaled to hold -Ncharge electrons
         (src/initialize.cpp:699-735); keep_phi: the next Poisson solve starts
@@ -246,6 +253,9 @@ class SCF:
         out = self.sq.density(self.veff, scf_iter=self.scf_iter, ncharge=self.ncharge)
         t1 = time.perf_counter()
         etot = self.EvaluateTotalEnergy(out["rho"], out["ebs"], out["eent"])
+        if self.prnt_scf:      # PrintSCF after the energy (src/scf.cpp:162-163)
+            from . import files
+            files.write_restart(self.prnt_scf, out["rho"], self.S.n)
         phi = self.PoissonSolver_AAJ(self.b + out["rho"], self.phi)
         vxc = self.ExchangeCorrelationPotential(out["rho"])
         veff_new = phi + vxc

@@ -110,6 +110,47 @@ def test_total_forces_on_gpu_match_log(al):
     scf.close()
 
 
+def test_scf_files_on_gpu(al, tmp_path):
+    """The SCF writes its density in the .restart layout every iteration
+    (prnt_scf); a new SCF started from that file (restart_scf) starts from
+    that density and, at the input's SCF tolerance, stops after one
+    iteration near the same state; the forces go through a .aout file
+    unchanged to its 14 decimals."""
+    import os
+    from sqdft_amd import files
+    from test_scf_oracle import AL_FORCES
+    S, _, _ = al
+    say("scf: .restart and .aout files")
+    p = str(tmp_path / "test_Al.restart")
+    scf = SCF(S)
+    scf.prnt_scf = p
+    out = scf.run()
+    assert out["converged"] and out["iterations"] == 9
+    back = files.read_restart(p, S.n, nproc=1)
+    assert np.max(np.abs(back - scf.rho)) <= 6e-15      # written to 14 decimals
+    f = scf.ForcesTotal()
+    types, _ = files.read_atoms_file(os.path.join(H.GOLDEN, "Al", "Al_eq.atoms"))
+    xyz, _ = S.atoms()
+    q = str(tmp_path / "test_Al.aout")
+    files.write_aout(q, types, xyz, f)
+    r = files.read_aout(q)
+    assert np.max(np.abs(r["forces"] - f)) <= 6e-15
+    assert np.max(np.abs(r["forces"] - AL_FORCES)) < 1e-4
+    scf.close()
+    scf2 = SCF(S)
+    scf2.restart_density(p, nproc=1)
+    ratio = scf2.rho / back      # rescaled to Ncharge electrons, like any starting density
+    assert np.ptp(ratio) < 1e-13 and abs(ratio[0] - 1.0) < 1e-6
+    scf2.initial_potential()
+    out2 = scf2.run()
+    say("  restarted SCF:", out2["iterations"], "iterations, SCF_err", out2["history"][0]["scf_err"],
+        "(from rho_at:", out["history"][0]["scf_err"], ") free energy", out2["free_energy"], out["free_energy"])
+    assert out2["converged"] and out2["iterations"] == 1
+    assert out2["history"][0]["scf_err"] < 1e-4 < out["history"][0]["scf_err"]
+    assert abs(out2["free_energy"] - out["free_energy"]) < 1e-4
+    scf2.close()
+
+
 def test_lih_scf_and_forces_on_gpu_match_log():
     """The reference's LiH test (two types, Li p projectors) through the
     C-ABI: the first Poisson solve, all 7 SCF iterations with Periodic Pulay

@@ -7,9 +7,11 @@ cell (96^3 FD nodes, 12th-order FD, nloc = 17, 35^3 cube, npl = 48,
 T = 58,000 K).  Batches are taken in the library's visiting order and cycle
 through the cell.  Veff and the projector tables are resident on the device;
 each step uploads Veff (7 MB) and returns the batch's moments, and its
-kernels dominate (99 %).  Then, once each: the whole density step of the cell
-(every batch, the Fermi level, rho) in FP64 and in mixed precision with the
-density error between them; an SCF iteration and the forces of the 216-atom
+kernels dominate (99 %).  The same batch steps in mixed precision (FP32
+storage) and with FP32 Lanczos passes only.  Then, once each: the whole
+density step of the cell (every batch, the Fermi level, rho) in FP64, mixed
+precision and FP32 Lanczos, with the density errors against FP64; an SCF
+iteration and the forces of the 216-atom
 LiH cell; SCF mixing and NVE steps on the reference's Al test input; the
 FP64 oracle on the host's cores.  Prints one JSON line.
 """
@@ -241,8 +243,16 @@ def main():
         msec, mper, _ = batch_steps(sqm, veff, order, B, 1, 4, "mixed")
         out["mixed"] = {"value": mper / msec, "unit": "nodes/s", "ms_per_step": 1e3 * msec,
                         "speedup_vs_fp64": sec / msec,
+                        "what": "FP32 storage of every vector, Veff and table; FP64 arithmetic and reductions",
                         "roofline": roofline(sqm, args.workload, "mixed", N)}
         sqm.close()
+        log("bench: mixed_lanczos batch steps")
+        sql = SpectralQuadrature(S, precision="mixed_lanczos", batch_nodes=B)
+        lsec, lper, _ = batch_steps(sql, veff, order, B, 1, 4, "mixed_lanczos")
+        out["mixed_lanczos"] = {"value": lper / lsec, "unit": "nodes/s", "ms_per_step": 1e3 * lsec,
+                                "speedup_vs_fp64": sec / lsec,
+                                "what": "the Lanczos passes as in mixed, the Chebyshev moments in FP64"}
+        sql.close()
     if not args.no_full:
         # the whole density step (every batch, global bounds, Fermi level, rho,
         # Ebs, Eent); both Fermi levels solved to 1e-12 so that the density
@@ -275,6 +285,18 @@ def main():
                 "rho_rel_err_l2_vs_fp64": float(np.linalg.norm(d32["rho"] - d64["rho"]) / np.linalg.norm(d64["rho"])),
                 "lambda_f_err": abs(d32["lambda_f"] - d64["lambda_f"]),
                 "ebs_rel_err": abs(d32["ebs"] - d64["ebs"]) / abs(d64["ebs"])}
+            log("bench: full mixed_lanczos density step")
+            rl = SpectralQuadrature(S, precision="mixed_lanczos", batch_nodes=B, fermi_tol=1e-12)
+            t0 = time.perf_counter()
+            dl = rl.density(veff, scf_iter=1)
+            tl = time.perf_counter() - t0
+            rl.close()
+            out["full_step"]["mixed_lanczos"] = {
+                "s": tl, "nodes_per_s": nn / tl, "speedup_vs_fp64": t64 / tl,
+                "rho_rel_err_vs_fp64": float(np.max(np.abs(dl["rho"] - d64["rho"])) / np.max(np.abs(d64["rho"]))),
+                "rho_rel_err_l2_vs_fp64": float(np.linalg.norm(dl["rho"] - d64["rho"]) / np.linalg.norm(d64["rho"])),
+                "lambda_f_err": abs(dl["lambda_f"] - d64["lambda_f"]),
+                "ebs_rel_err": abs(dl["ebs"] - d64["ebs"]) / abs(d64["ebs"])}
     if not args.no_scf:
         # one SCF iteration of the 216-atom LiH cell on its own pseudocharge:
         # SQ density, total energy, AAJ Poisson from the previous phi, LDA

@@ -56,6 +56,8 @@ extern "C" {
 #define SQG_FP64  0   /* every vector, stencil and reduction in FP64 */
 #define SQG_MIXED 1   /* FP32 vectors/stencil/Veff/projectors, FP64 reductions,
                          moments, coefficients, Lanczos tridiagonal and Fermi solve */
+#define SQG_MIXED_LANCZOS 2   /* the Lanczos phase as SQG_MIXED, the Chebyshev
+                         moments, forces and Veff derivatives as SQG_FP64 */
 
 typedef struct sqg_params {
     int n;                    /* FD nodes per direction of the periodic cube */
@@ -69,7 +71,7 @@ typedef struct sqg_params {
     double lanczos_tol;
     double fermi_tol;
     int fermi_alg;            /* 1 Brent, 2 Newton-Raphson */
-    int precision;            /* SQG_FP64 or SQG_MIXED */
+    int precision;            /* SQG_FP64, SQG_MIXED or SQG_MIXED_LANCZOS */
     int batch_nodes;          /* FD nodes per launch (0: all nodes, capped by memory) */
     int ckpt_interval;        /* adjoint checkpoint interval m: sqg_adjoint_veff keeps
                                  the recurrence pair every m steps and recomputes the

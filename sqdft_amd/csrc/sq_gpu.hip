@@ -70,6 +70,7 @@ struct sqg_ctx {
     std::vector<double> h_denom;
     DBuf img_type, img_nn, img_atom, ip_base, nlocc, nproj, udv_off, udv, inv_denom;
     DBuf ell_cnt, ell_u, ell_ip, ell_ip16;   // projector ELL over the extended grid
+    DBuf udv64, ell_u64, veff64;             // FP64 copies of udv, ell_u, veff (SQG_MIXED_LANCZOS)
     int ell_k = 0;                 // largest entry count of any point
     DBuf Dm, contrib, ncontrib;   // nonlocal forces
     // batch buffers
@@ -105,6 +106,9 @@ struct sqg_ctx {
     double last_ms[NEV] = {}, last_bytes[NEV] = {};
 
     size_t real_size() const { return p.precision == SQG_MIXED ? sizeof(float) : sizeof(double); }
+    // SQG_MIXED_LANCZOS: FP32 Lanczos passes, everything else in FP64 (the
+    // operator data are kept in both types)
+    bool split() const { return p.precision == SQG_MIXED_LANCZOS; }
     ~sqg_ctx()
     {
         for (auto e : pool) (void)hipEventDestroy(e);
@@ -224,13 +228,21 @@ int choose_batch_adj(sqg_ctx *c, int want, size_t ext)
     return std::max(B, 1);
 }
 
+// The operator data in the type a kernel of type T reads: the FP64 copies
+// when an FP64 phase runs in a context that stores them in FP32.
+template <typename T>
+T *op_data(sqg_ctx *c, DBuf &d, DBuf &d64)
+{
+    return (sizeof(T) == 8 && c->split()) ? d64.as<T>() : d.as<T>();
+}
+
 template <typename T>
 ZArgs<T> zargs(sqg_ctx *c, const int *dn, const int *dd)
 {
     ZArgs<T> A{};
     A.g = make_geo(c, dn, dd);
     A.nlv = c->NLv.as<T>();
-    A.veff = c->veff.as<T>();
+    A.veff = op_data<T>(c, c->veff, c->veff64);
     return A;
 }
 
@@ -250,7 +262,7 @@ PArgs<T> pargs(sqg_ctx *c, const int *dn, const int *dd, const T *V)
     A.img_nn = c->img_nn.as<int>();
     A.ip_base = c->ip_base.as<int>();
     A.udv_off = c->udv_off.as<long>();
-    A.udv = c->udv.as<T>();
+    A.udv = op_data<T>(c, c->udv, c->udv64);
     A.nlocc = c->nlocc.as<int>();
     A.nproj = c->nproj.as<int>();
     A.inv_denom = c->inv_denom.as<double>();
@@ -265,21 +277,22 @@ constexpr int PROJR_NT = 512, PROJR_RPI = 3;
 constexpr int PROJR_SLOTS[3] = {5, 10, 16};   // compiled slot counts (five keep FP64 out of scratch)
 constexpr int PROJM_SLOTS = 16;               // k_projm: one double per slot and lane
 struct ProjrShape { int nm, R, sz, mfma; };
-ProjrShape projr_shape(const sqg_ctx *c)
+// ts: bytes per stored value of the phase (4 or 8).
+ProjrShape projr_shape(const sqg_ctx *c, size_t ts)
 {
     if (c->max_nproj < 1 || c->max_nproj > 4 || c->n_ip == 0) return {0, 0, 0, 0};
     // p channels in FP64: k_projm on the matrix cores.  FP32 tables (mixed
     // precision) stay with k_projs: its VALU sums of FP32 operands ran at
     // 1.18 ms a pass on the 216-atom LiH batches against k_projm's 1.67 ms
     // (FP64 MFMAs fed by conversions), while in FP64 the two tie (2.3 ms).
-    if (c->max_nproj > 1 && c->p.precision == SQG_MIXED) return {0, 0, 0, 0};
+    if (c->max_nproj > 1 && ts == 4) return {0, 0, 0, 0};
     const int mfma = c->max_nproj > 1;
     int w = 0;
     for (int t = 0; t < c->n_typ; ++t)
         if (c->h_nproj[t] > 0) w = std::max(w, 2 * c->h_nlocc[t] + 1);
     if (mfma ? w > 16 : w * w > 64 * PROJR_RPI) return {0, 0, 0, 0};
     // eight members per workgroup; FP32 slabs of two planes fit the LDS
-    return {8, 0, c->p.precision == SQG_MIXED ? 2 : 1, mfma};
+    return {8, 0, ts == 4 ? 2 : 1, mfma};
 }
 
 // k_projr's slab capacity per plane and member: the union plane of a 4x2
@@ -312,15 +325,26 @@ int node_images(sqg_ctx *c, int B)
     L.overflow = c->flags.as<int>();
     L.glist = c->glist.as<int>();
     L.glist_cnt = c->glist_cnt.as<int>();
-    const ProjrShape ps = projr_shape(c);
+    // the projection shapes of the phases that will run on these lists (both
+    // stored types in SQG_MIXED_LANCZOS): each plans its register slots
+    const ProjrShape shapes[2] = {projr_shape(c, c->split() ? 4 : c->real_size()), projr_shape(c, 8)};
+    const int nsh = c->split() ? 2 : 1;
     hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, c->st, c->flags.as<int>() + 4, (size_t)1, 0);
-    if (ps.nm) {
-        L.pr_bad = c->flags.as<int>() + 4;
-        L.pr_nw = PROJR_NT / 64; L.pr_sz = ps.sz; L.pr_pu = projr_pu(c, ps); L.pr_mfma = ps.mfma;
-    }
     hipLaunchKernelGGL(k_node_images, dim3((B + 127) / 128), dim3(128), 0, c->st, L, B);
     const int ng = (B + NG - 1) / NG;
-    hipLaunchKernelGGL(k_group_images, dim3((ng + 63) / 64), dim3(64), 0, c->st, L, B);
+    bool any_nm = false, any_mfma = false;
+    for (int i = 0; i < nsh; ++i) {
+        const ProjrShape &ps = shapes[i];
+        LArgs Li = L;
+        if (ps.nm) {
+            Li.pr_bad = c->flags.as<int>() + 4;
+            Li.pr_nw = PROJR_NT / 64; Li.pr_sz = ps.sz; Li.pr_pu = projr_pu(c, ps); Li.pr_mfma = ps.mfma;
+            any_nm = true;
+            any_mfma |= ps.mfma != 0;
+        }
+        // a second launch rebuilds the same group lists and adds its slot plan
+        if (i == 0 || ps.nm) hipLaunchKernelGGL(k_group_images, dim3((ng + 63) / 64), dim3(64), 0, c->st, Li, B);
+    }
     std::vector<int> gc(ng + 1);
     HIPCHK(hipMemcpyAsync(gc.data(), c->glist_cnt.p, (size_t)ng * sizeof(int), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(gc.data() + ng, c->flags.as<int>() + 4, sizeof(int), hipMemcpyDeviceToHost, c->st));
@@ -330,9 +354,9 @@ int node_images(sqg_ctx *c, int B)
     // the smallest slot count that holds every group of the batch: k_projr
     // gives the same sums with any slot count, so batches stay bitwise alike
     c->projr_R = 0;
-    if (ps.nm && ps.mfma) {
+    if (any_mfma) {
         if (gc[ng] <= PROJM_SLOTS) c->projr_R = PROJM_SLOTS;
-    } else if (ps.nm) {
+    } else if (any_nm) {
         for (int r : PROJR_SLOTS)
             if (gc[ng] <= r) { c->projr_R = r; break; }
     }
@@ -419,7 +443,7 @@ void projr_go(sqg_ctx *c, dim3 grid, size_t shm, const PArgs<T> &P)
 template <typename T, int FORMED>
 bool launch_projr(sqg_ctx *c, dim3 grid, PArgs<T> &P)
 {
-    const ProjrShape ps = projr_shape(c);
+    const ProjrShape ps = projr_shape(c, sizeof(T));
     const size_t sa = ((size_t)std::max(c->gl_max, 1) * sizeof(ImgMeta) + 15) / 16 * 16;
     const int pu = projr_pu(c, ps);
     const size_t shm = sa + (size_t)pu * ps.nm * sizeof(T) * ps.sz;
@@ -479,7 +503,7 @@ apply:
     NArgs<T> N{};
     N.g = A.g;
     N.nl.cnt = c->ell_cnt.as<unsigned char>();
-    N.nl.u = c->ell_u.as<T>();
+    N.nl.u = op_data<T>(c, c->ell_u, c->ell_u64);
     N.nl.ip = c->ell_ip.as<int>();
     N.nl.ip16 = c->n_ip <= 65535 ? c->ell_ip16.as<unsigned short>() : nullptr;
     N.nl.E3 = (size_t)c->E * c->E * c->E;
@@ -621,7 +645,8 @@ int lanczos_batch(sqg_ctx *c, int B, long *passes_out)
 // Lanczos + Chebyshev moments for B nodes already in node_list.
 // Results land in chiB/zeeB/lmin/lmax/cnt (per batch) and mu_all (global).
 // Without lanczos the bounds of the last density step are reused.
-template <typename T>
+// TL: the stored type of the Lanczos passes, T: of the Chebyshev steps.
+template <typename TL, typename T>
 int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
 {
     const int *dn = c->node_list.as<int>();
@@ -636,7 +661,7 @@ int run_batch(sqg_ctx *c, int B, long *passes_out, bool lanczos)
     T *X = c->X.as<T>(), *Y = c->Y.as<T>();
     const double rs = (double)sizeof(T);
     if (lanczos) {
-        const int st = lanczos_batch<T>(c, B, passes_out);
+        const int st = lanczos_batch<TL>(c, B, passes_out);
         if (st) return st;
     } else {   // frozen bounds of the last density step
         hipLaunchKernelGGL(k_gather_bounds, dim3((B + 127) / 128), dim3(128), 0, c->st, B, dn,
@@ -721,14 +746,21 @@ int upload_veff(sqg_ctx *c, const double *veff)
 {
     const size_t N = (size_t)c->nn;
     DBuf tmp;
-    HIPCHK(c->veff.ensure(N * c->real_size()));
-    if (c->p.precision == SQG_MIXED) {
-        HIPCHK(tmp.ensure(N * sizeof(double)));
-        HIPCHK(hipMemcpyAsync(tmp.p, veff, N * sizeof(double), hipMemcpyHostToDevice, c->st));
-        hipLaunchKernelGGL((k_convert<float>), dim3(grid1(N, 256)), dim3(256), 0, c->st, tmp.as<double>(),
+    if (c->p.precision == SQG_MIXED || c->split()) {
+        HIPCHK(c->veff.ensure(N * sizeof(float)));
+        DBuf *dst = &tmp;
+        if (c->split()) {
+            HIPCHK(c->veff64.ensure(N * sizeof(double)));
+            dst = &c->veff64;
+        } else {
+            HIPCHK(tmp.ensure(N * sizeof(double)));
+        }
+        HIPCHK(hipMemcpyAsync(dst->p, veff, N * sizeof(double), hipMemcpyHostToDevice, c->st));
+        hipLaunchKernelGGL((k_convert<float>), dim3(grid1(N, 256)), dim3(256), 0, c->st, dst->as<double>(),
                            c->veff.as<float>(), N);
         HIPCHK(hipStreamSynchronize(c->st));
     } else {
+        HIPCHK(c->veff.ensure(N * sizeof(double)));
         HIPCHK(hipMemcpyAsync(c->veff.p, veff, N * sizeof(double), hipMemcpyHostToDevice, c->st));
     }
     return SQG_OK;
@@ -744,7 +776,7 @@ int ensure_node_state(sqg_ctx *c)
     return SQG_OK;
 }
 
-template <typename T>
+template <typename TL, typename T>
 int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int batch_cap, bool lanczos)
 {
     int done_n = 0;
@@ -753,7 +785,7 @@ int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int
         int st = ensure_batch(c, B);
         if (st) return st;
         HIPCHK(hipMemcpyAsync(c->node_list.p, nodes + done_n, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->st));
-        st = run_batch<T>(c, B, passes, lanczos);
+        st = run_batch<TL, T>(c, B, passes, lanczos);
         if (st) return st;
         if (lanczos)
             hipLaunchKernelGGL(k_scatter_batch, dim3((B + 127) / 128), dim3(128), 0, c->st, B,
@@ -770,8 +802,9 @@ int moments_for_nodes(sqg_ctx *c, const int *nodes, int count, long *passes, int
 int moments_dispatch(sqg_ctx *c, const int *nodes, int count, long *passes, int cap = 0, bool lanczos = true)
 {
     if (cap <= 0) cap = choose_batch(c, count);
-    if (c->p.precision == SQG_MIXED) return moments_for_nodes<float>(c, nodes, count, passes, cap, lanczos);
-    return moments_for_nodes<double>(c, nodes, count, passes, cap, lanczos);
+    if (c->p.precision == SQG_MIXED) return moments_for_nodes<float, float>(c, nodes, count, passes, cap, lanczos);
+    if (c->split()) return moments_for_nodes<float, double>(c, nodes, count, passes, cap, lanczos);
+    return moments_for_nodes<double, double>(c, nodes, count, passes, cap, lanczos);
 }
 
 // electron-count constraint g(lambda) (sq.cpp:1183-1228); mode 1 gives d/dlambda / bet
@@ -913,7 +946,7 @@ int forces_batch(sqg_ctx *c, int B, const int *hnodes, std::vector<double> &rec)
     F.list = c->list.as<int>(); F.list_cnt = c->list_cnt.as<int>();
     F.img_type = c->img_type.as<int>(); F.img_nn = c->img_nn.as<int>(); F.img_atom = c->img_atom.as<int>();
     F.nlocc = c->nlocc.as<int>(); F.nproj = c->nproj.as<int>();
-    F.udv_off = c->udv_off.as<long>(); F.udv = c->udv.as<T>(); F.inv_denom = c->inv_denom.as<double>();
+    F.udv_off = c->udv_off.as<long>(); F.udv = op_data<T>(c, c->udv, c->udv64); F.inv_denom = c->inv_denom.as<double>();
     for (int a = 0; a <= FDN; ++a) F.grad[a] = c->p.coeff_grad[a];
     F.contrib = c->contrib.as<double>(); F.ncontrib = c->ncontrib.as<int>(); F.overflow = flags + 2;
     if (c->max_nproj <= 1) hipLaunchKernelGGL((k_force_contract<T, 1>), dim3(B), dim3(256), 0, c->st, F);
@@ -1004,7 +1037,8 @@ int sqg_create(const sqg_params *p, sqg_ctx **out)
     if (p->npl <= 0 || p->npl % 4) return SQG_E_NPL;
     if (p->n < 1 || p->nloc < 0 || p->h <= 0) return SQG_E_ARG;
     if (2 * p->nloc + 1 > MAXM) return SQG_E_CUBE;
-    if (p->precision != SQG_FP64 && p->precision != SQG_MIXED) return SQG_E_ARG;
+    if (p->precision != SQG_FP64 && p->precision != SQG_MIXED && p->precision != SQG_MIXED_LANCZOS)
+        return SQG_E_ARG;
     // Veff is read through a buffer resource whose byte range is 32-bit
     if ((size_t)p->n * p->n * p->n * sizeof(double) >= 0x80000000u) return SQG_E_OVERFLOW;
     std::unique_ptr<sqg_ctx> c(new sqg_ctx());
@@ -1137,7 +1171,7 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
         }
     });
     c->ell_k = kmax;
-    const bool mixed = c->p.precision == SQG_MIXED;
+    const bool mixed = c->p.precision == SQG_MIXED || c->split();
     auto up = [&](DBuf &d, const void *src, size_t bytes) -> int {
         HIPCHK(d.ensure(bytes));
         if (bytes) HIPCHK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
@@ -1164,11 +1198,13 @@ int sqg_set_geometry(sqg_ctx *c, int n_typ, const int *nlocc, const int *nproj, 
         for (size_t i = 0; i < eu.size(); ++i) ef[i] = (float)eu[i];
         st |= up(c->udv, uf.data(), uf.size() * sizeof(float));
         st |= up(c->ell_u, ef.data(), ef.size() * sizeof(float));
-    } else {
+    }
+    if (!mixed || c->split()) {
+        DBuf &du = c->split() ? c->udv64 : c->udv, &de = c->split() ? c->ell_u64 : c->ell_u;
         std::vector<double> ud(udv, udv + off);
         if (ud.empty()) ud.push_back(0.0);
-        st |= up(c->udv, ud.data(), ud.size() * sizeof(double));
-        st |= up(c->ell_u, eu.data(), eu.size() * sizeof(double));
+        st |= up(du, ud.data(), ud.size() * sizeof(double));
+        st |= up(de, eu.data(), eu.size() * sizeof(double));
     }
     if (st) return SQG_E_HIP;
     // the nonlocal apply keeps every projection of its node in LDS (160 KB per CU)

@@ -24,7 +24,7 @@ import numpy as np
 from . import _lib as L
 from .system import System
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed_lanczos": 2}
 
 
 def node_order(n: int) -> np.ndarray:

@@ -183,15 +183,22 @@ def test_nonlocal_forces_match_oracle(cases, name):
     fm = sqm.ForcesNonLocal()
     say("mixed force err", np.max(np.abs(fm - fref)))
     assert np.max(np.abs(fm - fref)) < 1e-4
+    # FP32 Lanczos only: the force pass is FP64, with the FP32 passes' bounds
+    sql = SpectralQuadrature(S, precision="mixed_lanczos")
+    sql.density(veff, scf_iter=1)
+    fl = sql.ForcesNonLocal()
+    say("mixed_lanczos force err", np.max(np.abs(fl - fref)))
+    assert np.max(np.abs(fl - fref)) < 1e-6
 
 
 def test_density_batching_is_invisible(cases):
     S, osys, veff = cases["small_al"]
     say("batching")
-    a = SpectralQuadrature(S, precision="fp64").density(veff)
-    b = SpectralQuadrature(S, precision="fp64", batch_nodes=37).density(veff)
-    assert np.array_equal(a["rho"], b["rho"])
-    assert a["lambda_f"] == b["lambda_f"]
+    for prec in ("fp64", "mixed_lanczos"):
+        a = SpectralQuadrature(S, precision=prec).density(veff)
+        b = SpectralQuadrature(S, precision=prec, batch_nodes=37).density(veff)
+        assert np.array_equal(a["rho"], b["rho"]), prec
+        assert a["lambda_f"] == b["lambda_f"], prec
 
 
 def test_mixed_precision_density_error(cases):
@@ -211,6 +218,13 @@ def test_mixed_precision_density_error(cases):
     say("mixed rho rel err", err, "dlf", out["lambda_f"] - ref["lambda_f"])
     assert err < 2e-6
     assert out["lambda_f"] == pytest.approx(ref["lambda_f"], abs=1e-6)
+    # FP32 Lanczos passes, FP64 Chebyshev moments: the FP32 storage of the
+    # Chebyshev vectors is what costs the mixed mode its 1e-6
+    lo = SpectralQuadrature(S, precision="mixed_lanczos", fermi_tol=1e-12).density(veff)
+    errl = H.rel(lo["rho"], ref["rho"])
+    say("mixed_lanczos rho rel err", errl, "dlf", lo["lambda_f"] - ref["lambda_f"])
+    assert errl < 1e-6
+    assert lo["lambda_f"] == pytest.approx(ref["lambda_f"], abs=1e-8)
 
 
 def test_electron_count_at_config0_geometry():
@@ -237,7 +251,8 @@ def test_full_size_density_matches_oracle_fixture(name):
     cubes, npl 64) against the FP64 oracle's run on the same inputs
     (tests/golden/sq_<name>.npz, written by make_sq_fixtures.py): per-node
     rho, Lanczos counts, Fermi level, Ebs, Eent; mixed precision within its
-    stated 5e-6 (Fermi tolerance 1e-6 on both sides)."""
+    stated 5e-6, FP32 Lanczos with FP64 moments within 1e-6 (Fermi
+    tolerance 1e-6 on both sides)."""
     path = os.path.join(H.GOLDEN, f"sq_{name}.npz")
     if not os.path.exists(path):
         pytest.skip(f"{path} not generated (tests/golden/make_sq_fixtures.py {name})")
@@ -265,3 +280,10 @@ def test_full_size_density_matches_oracle_fixture(name):
     say(name, "mixed rho rel err", errm)
     assert errm < 5e-6
     assert mo["lambda_f"] == pytest.approx(float(fx["lambda_f"]), abs=2e-6)
+    sql = SpectralQuadrature(S, precision="mixed_lanczos")
+    lo = sql.density(veff, scf_iter=1)
+    sql.close()
+    errl = H.rel(lo["rho"][::k], fx["rho"])
+    say(name, "mixed_lanczos rho rel err", errl, "dlf", lo["lambda_f"] - float(fx["lambda_f"]))
+    assert errl < 1e-6
+    assert lo["lambda_f"] == pytest.approx(float(fx["lambda_f"]), abs=1e-6)

@@ -2,8 +2,10 @@
 //
 // Per SQ density step (ClenshawCurtisSpectralQuadrature, sq.cpp:377-834):
 //   for each batch of FD nodes (node-major cubes in HBM):
-//     image lists  -> Lanczos passes (projection + z-march + FP64 scalars/QL)
-//                  -> Chebyshev steps (projection + fused z-march recurrence)
+//     image lists  -> Lanczos passes (projections + nonlocal apply + z-march
+//                     + FP64 extremes of the tridiagonal)
+//                  -> Chebyshev steps (projections + nonlocal apply + fused
+//                     z-march recurrence and moment dots)
 //   Fermi level (Brent or Newton on the host, each g(lambda) one device sum)
 //   rho, Ebs, Eent, Fermi coefficients.
 #include "sqdft_gpu.h"

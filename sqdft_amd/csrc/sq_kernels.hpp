@@ -8,10 +8,12 @@
 //   Veff                [n^3] periodic grid (x fastest), read through wrapped indices
 //   projector ELL       over the extended grid (n + 2 nloc)^3: per point the
 //                       (image-projector id, U value) pairs that cover it
-// One workgroup owns one node cube: it marches through the cube along z,
-// keeping a 13-plane register window per column (z-neighbours), staging the
-// current plane in LDS (x/y-neighbours) and streaming each output plane to
-// HBM as soon as it is complete.
+// Two workgroups own one node cube, one per half of its rows: each marches
+// through the cube along z, keeping a 13-plane register window per column
+// (z-neighbours), staging the current plane in LDS (x/y-neighbours, with six
+// halo rows beyond its half) and streaming each output plane to HBM as soon
+// as it is complete.  The projections of 8 consecutive cubes (a 4x2 half
+// brick) are formed by one workgroup from their union, slab by slab.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -127,7 +127,7 @@ class NVE:
         self.a = np.zeros_like(self.x)            # the reference's forces are zero here
         self.step_count = 0
         n3 = S.n ** 3
-        self.rho_at_sum = np.zeros(n3)            # rho_at of every geometry so far (see DESIGN)
+        self.rho_at_sum = np.zeros(n3)            # rho_at of every geometry so far: the reference never clears it
         self.drho_dt, self.drho_2dt, self.drho_new = np.zeros(n3), np.zeros(n3), np.zeros(n3)
         self.r0, self.r1, self.r2 = np.zeros_like(self.x), np.zeros_like(self.x), np.zeros_like(self.x)
         self.history = []

@@ -40,7 +40,7 @@ def cases():
 
 
 @pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_lanczos"])
 def test_hsub_matches_oracle(cases, name, prec):
     S, osys, veff = cases[name]
     say("hsub", name, prec)
@@ -51,7 +51,8 @@ def test_hsub_matches_oracle(cases, name, prec):
     hv = sq.HsubTimesVec(veff, nodes, v).reshape(nodes.size, -1)
     for i, nd in enumerate(nodes):
         ref = osys.hsub(veff, int(nd), v[i])
-        assert H.rel(hv[i], ref) < (1e-13 if prec == "fp64" else 2e-6), (name, nd)
+        # mixed_lanczos applies H in FP64 (its FP64 copies of the operator data)
+        assert H.rel(hv[i], ref) < (2e-6 if prec == "mixed" else 1e-13), (name, nd)
 
 
 @pytest.mark.parametrize("name", ["small_al", "small_lih", "test_al", "test_lih"])

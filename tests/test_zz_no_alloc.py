@@ -2,7 +2,8 @@
 its per-call buffers in the library's static scratch array) against
 sqg_hsub_apply_batch on the same nodes and vectors, through the C-ABI: FP64
 bit for bit, mixed precision bit for bit (the same kernels on the same FP32
-copies), on two calls in a row, the second with fewer nodes; and the
+copies), FP32-Lanczos contexts bit for bit (H v in FP64 on the context's
+FP64 operator data), on two calls in a row, the second with fewer nodes; and the
 SQG_E_OVERFLOW answer when the scratch array is too small.  Device inputs
 and outputs are allocated through the same HIP runtime the library uses
 (/opt/rocm/lib/libamdhip64.so), so that one runtime owns every device
@@ -63,7 +64,7 @@ def noalloc(sq, veff_d, nodes, v_d, hv_d):
     return sq._lib.sqg_hsub_apply_batch_noalloc(sq._ctx, veff_d.p, L.iptr(nd), nd.size, v_d.p, hv_d.p)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed_lanczos"])
 @pytest.mark.parametrize("case", ["small_lih", "test_al"])
 def test_noalloc_matches_hsub(prec, case):
     S, _, _ = getattr(H, case)()

@@ -168,10 +168,25 @@ int tangent_batch(sqg_ctx *c, int s0, int B)
                        c->adj_in.as<double>(), c->zee_all.as<double>(), dr[1], rec);
     for (int k = 2; k <= K2; ++k) {
         cheb_step<T>(c, B, tr[(k - 1) % 3], tr[(k - 2) % 3], tr[k % 3], false);
-        cheb_step<T>(c, B, dr[(k - 1) % 3], dr[(k - 2) % 3], dr[k % 3], false);
-        hipLaunchKernelGGL((k_tan_update<T>), dim3(B), dim3(256), 0, c->st, c->p.n, c->p.nloc, cidx, k, K2, dn,
-                           c->adj_in.as<double>(), c->zee_all.as<double>(), dr[k % 3], (const T *)tr[k % 3],
-                           (const T *)tr[(k - 1) % 3], (const T *)dr[(k - 1) % 3], rec);
+        // d_k = 2 A d_{k-1} - d_{k-2} + 2 dA t_{k-1} and its dots in one z-march
+        PArgs<T> P = pargs<T>(c, dn, nullptr, dr[(k - 1) % 3]);
+        launch_proj<T, 0>(c, B, P);
+        ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+        Z.X = dr[(k - 1) % 3];
+        Z.Y = dr[(k - 2) % 3];
+        Z.O = dr[k % 3];
+        Z.chi = c->chi_all.as<double>();
+        Z.zee = c->zee_all.as<double>();
+        Z.chi_by_node = 1;
+        Z.first = 0;
+        Z.center = c->center.as<double>();
+        Z.dots = c->dots.as<double>();
+        Z.dV = c->adj_in.as<double>();
+        Z.tk = tr[k % 3];
+        Z.tkm = tr[(k - 1) % 3];
+        launch_zmarch<T, MODE_TAN>(c, B, Z, -1, 0.0);
+        hipLaunchKernelGGL(k_tan_rec, dim3((B + 127) / 128), dim3(128), 0, c->st, B, k, K2,
+                           c->center.as<double>(), c->dots.as<double>(), rec);
     }
     hipLaunchKernelGGL(k_tan_finish, dim3((B + 127) / 128), dim3(128), 0, c->st, B, K2, q4, c->p.npl,
                        1.0 / std::pow(c->p.h, 3), dn, (const double *)rec, c->Ci_all.as<double>(),

@@ -150,37 +150,18 @@ __global__ void k_tan_init(int B, int M3, int c, int K2, const int *node_list, c
     }
 }
 
-// d_k = D (holding 2 A d_{k-1} - d_{k-2}) + 2 dA t_{k-1}; one workgroup per node
-template <typename T>
-__global__ void k_tan_update(int n, int nloc, int c, int k, int K2, const int *node_list, const double *dV,
-                             const double *zee, T *D, const T *tk, const T *tkm, const T *dkm, double *rec)
+// rec[b][k] = (d_k[c], t_k.d_k, d_k.t_{k-1} + t_k.d_{k-1}) from the tangent
+// z-march step (MODE_TAN): the centre value and its two row halves' dots,
+// added in half order.
+__global__ void k_tan_rec(int B, int k, int K2, const double *center, const double *dots, double *rec)
 {
-    const int b = blockIdx.x;
-    const int nd = node_list[b];
-    const int M = 2 * nloc + 1, M2 = M * M, M3 = M2 * M;
-    const int ni = nd % n, nj = (nd / n) % n, nk = nd / (n * n);
-    const double ze = zee[nd];
-    const size_t base = (size_t)b * M3;
-    double v[2] = {0.0, 0.0};
-    for (int p = threadIdx.x; p < M3; p += blockDim.x) {
-        const int x = p % M, y = (p / M) % M, z = p / M2;
-        const int g = (wrapn(nk - nloc + z, n) * n + wrapn(nj - nloc + y, n)) * n + wrapn(ni - nloc + x, n);
-        const double da = dV[g] / ze;
-        const double tm = (double)tkm[base + p];
-        const T dk = (T)((double)D[base + p] + 2.0 * da * tm);
-        D[base + p] = dk;
-        const double t = (double)tk[base + p];
-        v[0] += t * (double)dk;
-        v[1] += (double)dk * tm + t * (double)dkm[base + p];
-    }
-    __shared__ double red[32];
-    block_sum<2>(v, red);
-    if (threadIdx.x == 0) {
-        double *r = rec + ((size_t)b * (K2 + 1) + k) * 3;
-        r[0] = (double)D[base + c];
-        r[1] = v[0];
-        r[2] = v[1];
-    }
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const double *d = dots + 4 * (size_t)b;
+    double *r = rec + ((size_t)b * (K2 + 1) + k) * 3;
+    r[0] = center[b];
+    r[1] = d[0] + d[2];
+    r[2] = d[1] + d[3];
 }
 
 // drho_pc = sum_j 2 C_j dmu_j with dmu_1 = d_1[c]/h^3, dmu_k = d_k[c]/h^3,

@@ -106,7 +106,8 @@ struct NL {
     int n_ip;
 };
 
-enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3 };   // CHEBD: CHEB + D column
+// CHEBD: CHEB + D column; TAN: the forward-mode step d_k = 2 A d_{k-1} - d_{k-2} + 2 dA t_{k-1}
+enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3, MODE_TAN = 4 };
 
 template <typename T>
 struct ZArgs {
@@ -133,6 +134,9 @@ struct ZArgs {
     const double *Ci;             // [nn][npl+1]
     int k, npl;
     double inv_h3;
+    // TAN: dA = diag(dV)/zee on the cube; dots become (t_k.d_k, d_k.t_{k-1} + t_k.d_{k-1})
+    const double *dV;             // [n^3] the direction
+    const T *tk, *tkm;            // [B][M^3] t_k and t_{k-1} of the forward recurrence
 };
 
 // ---------------------------------------------------------------- z-march
@@ -247,7 +251,8 @@ __host__ __device__ inline int zhalf_rows(int M, int h) { return h == 0 ? (M + 1
 template <typename T, int MODE>
 __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
 {
-    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD;
+    constexpr bool TAN = MODE == MODE_TAN;
+    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD || TAN;
     constexpr bool YQ = MODE == MODE_LANCZOS && sizeof(T) == 8;
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
@@ -289,6 +294,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     const __amdgpu_buffer_rsrc_t rN = brsrc(A.nlv + base, vb);
     const __amdgpu_buffer_rsrc_t rD = brsrc(MODE == MODE_CHEBD ? Db : Yb, vb);
     const __amdgpu_buffer_rsrc_t rV = brsrc(A.veff, (size_t)n * n * n * TS);
+    const __amdgpu_buffer_rsrc_t rTk = brsrc(TAN ? A.tk + base : Yb, vb), rTm = brsrc(TAN ? A.tkm + base : Yb, vb);
+    const __amdgpu_buffer_rsrc_t rDV = brsrc(TAN ? (const void *)A.dV : (const void *)Yb,
+                                             TAN ? (size_t)n * n * n * sizeof(double) : vb);
 
     const int col = py * M + x0;                 // offset of the column in a plane
     const int hcol = hy * M + hx;                // offset of the halo point in a plane
@@ -355,6 +363,8 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     struct PW {
         T ve, nl, pv, dv;   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
         T hx, hyv;          // halo point: the two stored vectors
+        T tt, tm;           // TAN: t_k, t_{k-1}
+        double dvv;         // TAN: the direction dV at the point
     };
     auto fill_pw = [&](int z, PW &P) {
         const unsigned o = voff(z);
@@ -367,6 +377,14 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         const unsigned ho = hoff(z);
         P.hx = bload<T>(rX, ho);
         P.hyv = MODE == MODE_LANCZOS ? bload<T>(rY, ho) : (T)0;
+        if (TAN) {
+            P.tt = bload<T>(rTk, o);
+            P.tm = bload<T>(rTm, o);
+            P.dvv = bload<double>(rDV, (act && z < M) ? (unsigned)vo * 8u : OOB);
+        } else {
+            P.tt = P.tm = (T)0;
+            P.dvv = 0.0;
+        }
     };
 
     // window: w[s] holds plane z - 6 + s during step z; FP64 LANCZOS: yq[j]
@@ -424,9 +442,20 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
                 if (A.first) o = (hv - chi * cv) * izee;
                 else o = (hv - chi * cv) * two_izee - (R)S.pv;
                 const T os = (T)o;   // the stored value is the one reduced and projected
-                Yob[ozp] = os;
-                acc[0] += (double)os * (double)os;
-                acc[1] += (double)os * cv;
+                if (TAN) {
+                    // d_k = (2 A d_{k-1} - d_{k-2}) + 2 dA t_{k-1}; dots t_k.d_k and
+                    // d_k.t_{k-1} + t_k.d_{k-1} (d_{k-1} is this point's input)
+                    const T dn = (T)((double)os + 2.0 * (S.dvv * izee) * (double)S.tm);
+                    Yob[ozp] = dn;
+                    acc[0] += (double)S.tt * (double)dn;
+                    acc[1] += (double)dn * (double)S.tm + (double)S.tt * cv;
+                    if (z == nloc && py == nloc && x0 == nloc) sh_center = (double)dn;
+                } else {
+                    Yob[ozp] = os;
+                    acc[0] += (double)os * (double)os;
+                    acc[1] += (double)os * cv;
+                    if (z == nloc && py == nloc && x0 == nloc) sh_center = (double)os;
+                }
                 if (MODE == MODE_CHEBD) {
                     if (A.first) {   // D = C0 t0/h^3 + C1 t1/h^3 with t0 = e_c
                         const bool ctr = z == nloc && py == nloc && x0 == nloc;
@@ -435,7 +464,6 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
                         Db[ozp] = (T)((R)S.dv + (R)os * dk);
                     }
                 }
-                if (z == nloc && py == nloc && x0 == nloc) sh_center = (double)os;
             } else {  // LANCZOS: r' = H v - beta_k v_{k-1}; store v and r'
                 const T q = (T)(hv - bp * (R)S.pv);
                 A.XO[base + ozp] = q;

@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--no-full", action="store_true", help="skip the whole-cell density steps")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--mixing-workload", default="test_Al", choices=["test_Al"] + sorted(WORKLOADS),
+                    help="cell of the SCF mixing comparison (Periodic Pulay vs response mixing to 1e-6)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the steps, run one plain device copy of this size (MiB) so that "
                          "counter runs can scale FETCH_SIZE/WRITE_SIZE by a kernel of known traffic")
@@ -338,20 +340,27 @@ def main():
                                    "then local and nonlocal forces at that state"}
         # SCF iterations to SCF_err 1e-6 on the reference's Al test input:
         # its Periodic Pulay mixing vs Newton steps from the adjoint response
-        log("bench: scf mixing on the Al test input")
         from sqdft_amd.system import System
         A = System.from_files(os.path.join(ROOT, "tests", "golden", "Al"), "test_Al")
+        if args.mixing_workload == "test_Al":
+            log("bench: scf mixing on the Al test input")
+            MX, mx_what = A, "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6"
+        else:
+            log(f"bench: scf mixing on {args.mixing_workload}")
+            MX = configs.build(args.mixing_workload, seed=0)
+            mx_what = f"{args.mixing_workload}: {WORKLOADS[args.mixing_workload]}, scf_tol 1e-6, 40 iterations at most"
         mix = {}
         for name, kw in (("pulay", {}), ("response", {"mixing": "response"})):
-            sc = SCF(A)
+            sc = SCF(MX)
             t0 = time.perf_counter()
-            r = sc.run(scf_tol=1e-6, **kw)
+            r = sc.run(scf_tol=1e-6, progress=lambda i, e: log(f"bench: {name} iteration {i} SCF_err {e:.3e}"),
+                       **kw)
             mix[name] = {"iterations": r["iterations"], "converged": r["converged"],
                          "response_products": r["response_products"], "s": time.perf_counter() - t0,
-                         "free_energy": r["free_energy"]}
+                         "free_energy": r["free_energy"],
+                         "scf_err": [h["scf_err"] for h in r["history"]]}
             sc.close()
-        out["scf_mixing"] = {"workload": "reference tests/Al input (4 atoms, 10^3 nodes, npl 28), scf_tol 1e-6",
-                             **mix}
+        out["scf_mixing"] = {"workload": mx_what, **mix}
         log("bench: NVE steps on the Al test input")
         from sqdft_amd.md import NVE
         sc = SCF(A)

@@ -267,13 +267,14 @@ class SCF:
                 "poisson": self.poisson_log[-1], "t_sq": t1 - t0, "t_rest": t2 - t1}
 
     def run(self, mixing: str = "pulay", scf_tol: float | None = None, krylov: int = 4,
-            products: str = "adjoint") -> dict:
+            products: str = "adjoint", progress=None) -> dict:
         """SCF loop to scf_tol (SCF_iteration, src/scf.cpp:84-330): per
         iteration the SQ density, energy, Poisson and Vxc, then the Veff
         update.  mixing="pulay": the reference's Veff_old + beta_scf res with
         an Anderson step every p_scf iterations over the last m_scf
         differences.  mixing="response": the Newton step of response_step
-        (no counterpart in the reference)."""
+        (no counterpart in the reference).  progress(iteration, scf_err), when
+        given, is called after every iteration."""
         if mixing not in ("pulay", "response"):
             raise ValueError(f"unknown mixing {mixing!r}")
         pp = dict(self.pp)
@@ -317,6 +318,8 @@ class SCF:
                             "poisson": last["poisson"], "fermi_iters": last["stats"]["fermi_iters"],
                             "lanczos0": last["stats"]["lanczos_iters_node0"], "brent": last["stats"]["fermi_iters"],
                             "t_sq": last["t_sq"], "t_rest": last["t_rest"], "gmres_res": gres})
+            if progress is not None:
+                progress(count, err)
             count += 1
         return {"iterations": count - 1, "converged": err <= pp["scf_tol"], "free_energy": last["etot"],
                 "lambda_f": last["lambda_f"], "history": history, "response_products": n_prod}

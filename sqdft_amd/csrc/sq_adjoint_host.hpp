@@ -113,7 +113,27 @@ int adjoint_batch(sqg_ctx *c, int z0, int nz, const AdjPlan &P)
         auto sp = [&](int k) -> T * { return k - k0 <= 1 ? pair(j, k - k0) : seg(k - k0); };
         for (int kk = k0 + 1; kk < top; ++kk) cheb_step<T>(c, B, sp(kk), sp(kk - 1), sp(kk + 1), false);
         for (int k = khi; k >= k0 + 1; --k) {
-            if (k < K2) cheb_step<T>(c, B, tb(k + 1), tb(k + 2), tb(k), false);
+            if (k < K2) {   // the step and its source in one z-march
+                const int *dn = c->node_list.as<int>();
+                PArgs<T> P = pargs<T>(c, dn, nullptr, tb(k + 1));
+                launch_proj<T, 0>(c, B, P);
+                ZArgs<T> Z = zargs<T>(c, dn, nullptr);
+                Z.X = tb(k + 1);
+                Z.Y = tb(k + 2);
+                Z.O = tb(k);
+                Z.chi = c->chi_all.as<double>();
+                Z.zee = c->zee_all.as<double>();
+                Z.chi_by_node = 1;
+                Z.first = 0;
+                Z.Ci = c->Ci_all.as<double>();
+                Z.wt = c->adj_wt.as<double>();
+                Z.k = k; Z.npl = c->p.npl; Z.K2 = K2; Z.q4 = q4;
+                Z.inv_h3 = inv_h3;
+                Z.tk = sp(k); Z.tkm = sp(k - 1); Z.tkp = sp(k + 1);
+                Z.G = G;
+                launch_zmarch<T, MODE_ADJ>(c, B, Z, -1, 0.0);
+                continue;
+            }
             AdjArgs<T> A{};
             A.M3 = (int)M3; A.c = cidx; A.k = k; A.K2 = K2; A.q4 = q4; A.npl = c->p.npl;
             A.has_mv = k < K2;

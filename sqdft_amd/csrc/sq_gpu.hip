@@ -557,9 +557,9 @@ void launch_zmarch(sqg_ctx *c, int B, const ZArgs<T> &A, int which, double alg_b
         if (e0) (void)hipEventRecord(e0, c->st);
     }
     // FP32 workgroups of up to 640 threads (M <= 35) are held to 80 VGPRs so
-    // that two share a CU; FP64 ones (and the FP32 force and tangent passes,
-    // with their extra vectors) would spill there and run one per CU
-    if constexpr (sizeof(T) == 4 && MODE != MODE_CHEBD && MODE != MODE_TAN) {
+    // that two share a CU; FP64 ones (and the FP32 force and derivative
+    // passes, with their extra vectors) would spill there and run one per CU
+    if constexpr (sizeof(T) == 4 && MODE != MODE_CHEBD && MODE != MODE_TAN && MODE != MODE_ADJ) {
         if (zmarch_threads(c->M) <= 640) {
             zmarch_k<T, MODE, 640>(c, B, A, shm);
             return finish_zmarch(c, which, e0, e1, alg_bytes);

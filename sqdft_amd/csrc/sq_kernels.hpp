@@ -106,8 +106,9 @@ struct NL {
     int n_ip;
 };
 
-// CHEBD: CHEB + D column; TAN: the forward-mode step d_k = 2 A d_{k-1} - d_{k-2} + 2 dA t_{k-1}
-enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3, MODE_TAN = 4 };
+// CHEBD: CHEB + D column; TAN: the forward-mode step d_k = 2 A d_{k-1} - d_{k-2} + 2 dA t_{k-1};
+// ADJ: the reverse step tb_k = 2 A tb_{k+1} - tb_{k+2} + g_k with G += 2 tb_{k+1} t_k
+enum { MODE_HSUB = 0, MODE_CHEB = 1, MODE_LANCZOS = 2, MODE_CHEBD = 3, MODE_TAN = 4, MODE_ADJ = 5 };
 
 template <typename T>
 struct ZArgs {
@@ -137,6 +138,12 @@ struct ZArgs {
     // TAN: dA = diag(dV)/zee on the cube; dots become (t_k.d_k, d_k.t_{k-1} + t_k.d_{k-1})
     const double *dV;             // [n^3] the direction
     const T *tk, *tkm;            // [B][M^3] t_k and t_{k-1} of the forward recurrence
+    // ADJ (k < K2): the source g_k of dL/dmu (b_j = 2 C_j wt, Ci at k and the
+    // doubled moments) and the gradient accumulator G
+    const T *tkp;                 // [B][M^3] t_{k+1}
+    T *G;                         // [B][M^3]
+    const double *wt;             // [nn] dL/drho weights (after the Fermi shift)
+    int K2, q4;
 };
 
 // ---------------------------------------------------------------- z-march
@@ -252,7 +259,8 @@ template <typename T, int MODE>
 __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
 {
     constexpr bool TAN = MODE == MODE_TAN;
-    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD || TAN;
+    constexpr bool ADJ = MODE == MODE_ADJ;
+    constexpr bool CH = MODE == MODE_CHEB || MODE == MODE_CHEBD || TAN || ADJ;
     constexpr bool YQ = MODE == MODE_LANCZOS && sizeof(T) == 8;
     constexpr unsigned TS = sizeof(T);
     const Geo &G = A.g;
@@ -294,7 +302,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     const __amdgpu_buffer_rsrc_t rN = brsrc(A.nlv + base, vb);
     const __amdgpu_buffer_rsrc_t rD = brsrc(MODE == MODE_CHEBD ? Db : Yb, vb);
     const __amdgpu_buffer_rsrc_t rV = brsrc(A.veff, (size_t)n * n * n * TS);
-    const __amdgpu_buffer_rsrc_t rTk = brsrc(TAN ? A.tk + base : Yb, vb), rTm = brsrc(TAN ? A.tkm + base : Yb, vb);
+    const __amdgpu_buffer_rsrc_t rTk = brsrc((TAN || ADJ) ? A.tk + base : Yb, vb);
+    const __amdgpu_buffer_rsrc_t rTm = brsrc((TAN || ADJ) ? A.tkm + base : Yb, vb);
+    const __amdgpu_buffer_rsrc_t rTp = brsrc(ADJ ? A.tkp + base : Yb, vb), rG = brsrc(ADJ ? A.G + base : Yb, vb);
     const __amdgpu_buffer_rsrc_t rDV = brsrc(TAN ? (const void *)A.dV : (const void *)Yb,
                                              TAN ? (size_t)n * n * n * sizeof(double) : vb);
 
@@ -333,6 +343,27 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     if (MODE == MODE_LANCZOS) {
         ca = (T)A.ca[b]; ib = (T)A.ib[b]; bp = A.bprev[b];
     }
+    // ADJ: b_j = 2 C_j wt (dL/dmu_j) for the terms of g_k; the centre term
+    // carries b_k (and, at k = 1, minus the b_{2k'-1} of every doubled odd
+    // moment, each of which subtracts t_1[c])
+    double a_gc = 0.0, a_b2k = 0.0, a_b2km = 0.0, a_b2kp = 0.0;
+    bool adj_hi = false, adj_hp = false;
+    if (ADJ) {
+        const double *ci = A.Ci + (size_t)node * (A.npl + 1);
+        const double wtn = A.wt[node];
+        auto be = [&](int j) { return 2.0 * ci[j] * wtn; };
+        const int k = A.k;
+        a_gc = be(k) * A.inv_h3;
+        if (k == 1) {
+            double so = 0.0;
+            for (int kk = A.q4 + 1; kk <= A.K2; ++kk) so += be(2 * kk - 1);
+            a_gc -= so * A.inv_h3;
+        }
+        adj_hi = k > A.q4;
+        adj_hp = k + 1 > A.q4 && k + 1 <= A.K2;
+        if (adj_hi) { a_b2k = be(2 * k); a_b2km = be(2 * k - 1); }
+        if (adj_hp) a_b2kp = be(2 * k + 1);
+    }
     const bool cheb_prev = CH && !A.first;
     const bool dacc = MODE == MODE_CHEBD && !A.first;
 
@@ -363,8 +394,9 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
     struct PW {
         T ve, nl, pv, dv;   // Veff, nonlocal term, t_{k-1} / v_{k-1}, D
         T hx, hyv;          // halo point: the two stored vectors
-        T tt, tm;           // TAN: t_k, t_{k-1}
+        T tt, tm;           // TAN, ADJ: t_k, t_{k-1}
         double dvv;         // TAN: the direction dV at the point
+        T tp, gv;           // ADJ: t_{k+1}, G
     };
     auto fill_pw = [&](int z, PW &P) {
         const unsigned o = voff(z);
@@ -382,9 +414,12 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
             P.tm = bload<T>(rTm, o);
             P.dvv = bload<double>(rDV, (act && z < M) ? (unsigned)vo * 8u : OOB);
         } else {
-            P.tt = P.tm = (T)0;
+            P.tt = ADJ ? bload<T>(rTk, o) : (T)0;
+            P.tm = ADJ ? bload<T>(rTm, adj_hi ? o : OOB) : (T)0;
             P.dvv = 0.0;
         }
+        P.tp = ADJ ? bload<T>(rTp, adj_hp ? o : OOB) : (T)0;
+        P.gv = ADJ ? bload<T>(rG, o) : (T)0;
     };
 
     // window: w[s] holds plane z - 6 + s during step z; FP64 LANCZOS: yq[j]
@@ -442,7 +477,18 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
                 if (A.first) o = (hv - chi * cv) * izee;
                 else o = (hv - chi * cv) * two_izee - (R)S.pv;
                 const T os = (T)o;   // the stored value is the one reduced and projected
-                if (TAN) {
+                if (ADJ) {
+                    // tb_k = (2 A tb_{k+1} - tb_{k+2}) + g_k and G += 2 tb_{k+1} t_k
+                    // (tb_{k+1} is this point's input), in the terms' order of
+                    // the elementwise step k_adj_update
+                    const bool ctr = z == nloc && py == nloc && x0 == nloc;
+                    double g = ctr ? a_gc : 0.0;
+                    const double tkv = (double)S.tt;
+                    if (adj_hi) g += (4.0 * a_b2k * tkv + 2.0 * a_b2km * (double)S.tm) * A.inv_h3;
+                    if (adj_hp) g += 2.0 * a_b2kp * (double)S.tp * A.inv_h3;
+                    Yob[ozp] = (T)((double)os + g);
+                    A.G[base + ozp] = (T)((double)S.gv + 2.0 * cv * tkv);
+                } else if (TAN) {
                     // d_k = (2 A d_{k-1} - d_{k-2}) + 2 dA t_{k-1}; dots t_k.d_k and
                     // d_k.t_{k-1} + t_k.d_{k-1} (d_{k-1} is this point's input)
                     const T dn = (T)((double)os + 2.0 * (S.dvv * izee) * (double)S.tm);
@@ -488,7 +534,7 @@ __device__ __forceinline__ void zmarch_body(const ZArgs<T> &A)
         step(z, SA, SB);
         if (z + 1 < M) step(z + 1, SB, SA);
     }
-    if (MODE == MODE_HSUB) return;
+    if (MODE == MODE_HSUB || ADJ) return;
     block_sum<4>(acc, red);
     if (tid == 0) {
         const bool has_ctr = nloc >= y0 && nloc < y0 + ny;

@@ -40,7 +40,7 @@ def adj_case():
     veff = smooth_veff(S)
     rng = np.random.Generator(np.random.PCG64(3))
     w = rng.normal(size=S.nn)
-    say("config 3: density step (fp64, fermi_tol 1e-14)")
+    say("al32 adjoint: density step (fp64, fermi_tol 1e-14)")
     sq = SpectralQuadrature(S, precision="fp64", ckpt_interval=8, fermi_tol=1e-14)
     d = sq.density(veff, scf_iter=1)
     t0 = time.perf_counter()
@@ -50,9 +50,9 @@ def adj_case():
     sq.close()
 
 
-def test_config3_adjoint_vs_forward_mode(adj_case):
+def test_al32_adjoint_vs_forward_mode(adj_case):
     S, veff, w, sq, d, g64, rng = adj_case
-    say("config 3: 8 forward-mode directions")
+    say("al32 adjoint: 8 forward-mode directions")
     rng = np.random.Generator(np.random.PCG64(31))
     for i in range(8):
         dv = rng.normal(size=S.nn)
@@ -62,12 +62,12 @@ def test_config3_adjoint_vs_forward_mode(adj_case):
         assert lhs == pytest.approx(rhs, rel=1e-10, abs=1e-12 * np.abs(g64).sum() * np.abs(dv).max())
 
 
-def test_config3_adjoint_vs_central_differences(adj_case):
+def test_al32_adjoint_vs_central_differences(adj_case):
     S, veff, w, sq, d, g64, rng = adj_case
     lf = d["lambda_f"]
     eps = 1e-5
     nodes = np.random.Generator(np.random.PCG64(32)).choice(S.nn, size=64, replace=False)
-    say("config 3: central differences on 64 nodes")
+    say("al32 adjoint: central differences on 64 nodes")
     worst = 0.0
     gmax = np.abs(g64).max()
     for node in nodes:
@@ -81,11 +81,11 @@ def test_config3_adjoint_vs_central_differences(adj_case):
     say(f"  worst relative difference {worst:.2e}")
 
 
-def test_config3_mixed_adjoint(adj_case):
+def test_al32_mixed_adjoint(adj_case):
     # FP32 vectors, FP64 arithmetic and accumulation; the bounds chi, zee of
     # the mixed step's own Lanczos phase
     S, veff, w, sq, d, g64, rng = adj_case
-    say("config 3: mixed adjoint")
+    say("al32 adjoint: mixed adjoint")
     sqm = SpectralQuadrature(S, precision="mixed", ckpt_interval=8, fermi_tol=1e-14)
     sqm.density(veff, scf_iter=1)
     t0 = time.perf_counter()
@@ -111,13 +111,13 @@ def al864():
     assert S.nn == 96 ** 3
     veff = smooth_veff(S)
     nodes = np.sort(np.random.Generator(np.random.PCG64(41)).choice(S.nn, size=24, replace=False)).astype(np.int32)
-    say(f"config 4: built in {time.perf_counter() - t0:.1f} s")
+    say(f"al864: built in {time.perf_counter() - t0:.1f} s")
     return S, veff, nodes
 
 
-def test_config4_node_subset_vs_oracle(al864):
+def test_al864_node_subset_vs_oracle(al864):
     S, veff, nodes = al864
-    say("config 4: 24 seeded nodes on the GPU (batch 8192) and in the oracle")
+    say("al864: 24 seeded nodes on the GPU (batch 8192) and in the oracle")
     sq = SpectralQuadrature(S, precision="fp64", batch_nodes=8192)
     g = sq.node_moments(veff, nodes)
     sq.close()
@@ -132,9 +132,9 @@ def test_config4_node_subset_vs_oracle(al864):
         assert H.rel(g["mu"][i], ref["mu"][i]) < 1e-10, int(nodes[i])
 
 
-def test_config4_batch_size_invariance(al864):
+def test_al864_batch_size_invariance(al864):
     S, veff, nodes = al864
-    say("config 4: the subset in batches of 8192 and of 5")
+    say("al864: the subset in batches of 8192 and of 5")
     a = SpectralQuadrature(S, precision="fp64", batch_nodes=8192)
     ga = a.node_moments(veff, nodes)
     a.close()
@@ -145,9 +145,9 @@ def test_config4_batch_size_invariance(al864):
         assert np.array_equal(ga[k], gb[k]), k
 
 
-def test_config4_full_density_step(al864):
+def test_al864_full_density_step(al864):
     S, veff, nodes = al864
-    say("config 4: full FP64 density step in 8192-node batches")
+    say("al864: full FP64 density step in 8192-node batches")
     sq = SpectralQuadrature(S, precision="fp64", batch_nodes=8192)
     t0 = time.perf_counter()
     d = sq.density(veff, scf_iter=1)

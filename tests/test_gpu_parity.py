@@ -228,7 +228,7 @@ def test_mixed_precision_density_error(cases):
     assert lo["lambda_f"] == pytest.approx(ref["lambda_f"], abs=1e-8)
 
 
-def test_electron_count_at_config0_geometry():
+def test_electron_count_at_al32_geometry():
     # size-independent property on the 32-atom Al cell (n=32, nloc=17):
     # sum rho h^3 equals the electron count to the Fermi tolerance
     S = configs.build("al32", seed=0)
